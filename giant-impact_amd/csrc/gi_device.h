@@ -1,0 +1,102 @@
+// gi_device.h -- device-side types and the exact pair law shared by the HIP
+// kernels of the Giant-Impact step loop (gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace gi {
+
+// Constants of one simulation: the pair law of forceCollisionKernalConstantsStruct
+// (Code/ImpactCode/impactAddTime.cu:4-20) without the material split.
+struct PairLaw {
+    float GMassFeFe, GMassFeSi, KFeFe, KSiSi, KFeSi, KRFeFe, KRSiSi, KRFeSi, KRMix;
+    float ShellBreakFe, ShellBreakSi, ShellBreakFeSi1, ShellBreakFeSi2;
+};
+
+// One simulation inside a packed particle array.  Particles [start, end);
+// iron is [start, fe_end) (material by index, as the reference's id < NFe and
+// boarder tests).  Partial sums of the all-pairs pass for particle i and
+// j-chunk c live at partial[partial_base + c * (end - start) + (i - start)].
+struct SimDesc {
+    int start, fe_end, end;
+    int nchunk;
+    int partial_base;
+    int active;
+    int pad0, pad1;
+    PairLaw law;
+    float dt, dt_over_mass_fe, dt_over_mass_si;
+    float pad2;
+};
+
+// One wavefront of work for the all-pairs pass: 64*R i-particles starting at
+// i0 of simulation sim, against j in [j0, j1) (all of one material: jfe).
+struct GravTask {
+    int i0, sim, j0, j1, chunk, jfe, pad0, pad1;
+};
+
+// Added to r^2 in the all-pairs pass so the self pair (dp = 0) contributes
+// 0 * finite instead of 0 * inf.  For any pair with r^2 > 2^-40 it is below
+// half an ulp of every partial sum of r^2 and changes nothing.
+constexpr float kSelfEps = 1e-20f;
+
+// Hash of an integer cell (cell size 1 = the element diameter) and a
+// simulation id.
+__device__ __forceinline__ unsigned cell_hash(int cx, int cy, int cz, int sim) {
+    return ((unsigned)cx * 73856093u) ^ ((unsigned)cy * 19349663u) ^ ((unsigned)cz * 83492791u) ^
+           ((unsigned)sim * 2654435761u);
+}
+
+__device__ __forceinline__ int cell_coord(float x) {
+    float f = floorf(x);
+    f = fminf(fmaxf(f, -1.0e9f), 1.0e9f);  // NaN -> -1e9 (fmaxf drops NaN)
+    return (int)f;
+}
+
+// The reference's contact expression "G - K*(1.0 - r2)": the double literal
+// promotes it to FP64 and the result is narrowed to float
+// (impactAddTime.cu:306,314-315,326,334-335,343-344,355,363-364).  The
+// library is built with -ffp-contract=off, so this is mul then sub.
+__device__ __forceinline__ float contact_mag(float G, float K, float r2) {
+    return (float)((double)G - (double)K * (1.0 - (double)r2));
+}
+
+// Pair force magnitude of getForcesCollisionSingleGPU (impactAddTime.cu:298-366)
+// for a pair already known to be in contact (r < 1).  test = number of irons.
+// The receding test (int)(dp.dv) > 0 holds exactly when dp.dv >= 1.0f
+// (truncation toward zero; NaN converts to 0).
+__device__ __forceinline__ float contact_pair_mag(const PairLaw& L, int test, float r, float r2,
+                                                  float dot) {
+    const bool receding = dot >= 1.0f;
+    if (test == 0) {
+        if (L.ShellBreakSi <= r) return contact_mag(1.0f, L.KSiSi, r2);
+        return contact_mag(1.0f, receding ? L.KRSiSi : L.KSiSi, r2);
+    } else if (test == 1) {
+        if (L.ShellBreakFeSi1 <= r) return contact_mag(L.GMassFeSi, L.KFeSi, r2);
+        if (L.ShellBreakFeSi2 <= r) return contact_mag(L.GMassFeSi, receding ? L.KRMix : L.KFeSi, r2);
+        return contact_mag(L.GMassFeSi, receding ? L.KRFeSi : L.KFeSi, r2);
+    } else {
+        if (L.ShellBreakFe <= r) return contact_mag(L.GMassFeFe, L.KFeFe, r2);
+        return contact_mag(L.GMassFeFe, receding ? L.KRFeFe : L.KFeFe, r2);
+    }
+}
+
+// Gravity product G mi mj of a material pair as the force kernel uses it.
+__device__ __forceinline__ float gravity_product(const PairLaw& L, int test) {
+    return test == 0 ? 1.0f : (test == 1 ? L.GMassFeSi : L.GMassFeFe);
+}
+__device__ __forceinline__ float approach_k(const PairLaw& L, int test) {
+    return test == 0 ? L.KSiSi : (test == 1 ? L.KFeSi : L.KFeFe);
+}
+
+// r^-3 as the all-pairs pass computes it (one hardware reciprocal square
+// root, two multiplies) from r^2 + eps accumulated with FMAs.
+__device__ __forceinline__ float fast_r2(float dx, float dy, float dz) {
+    return __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, __builtin_fmaf(dx, dx, kSelfEps)));
+}
+__device__ __forceinline__ float fast_inv_r3(float r2) {
+    float ri = __builtin_amdgcn_rsqf(r2);
+    return ri * ri * ri;
+}
+
+}  // namespace gi

@@ -1,0 +1,556 @@
+// gi_kernels.hip -- HIP kernels of the Giant-Impact step loop for gfx950.
+//
+// The reference evaluates the Greenspan Fe/Si force in one all-pairs kernel
+// whose inner loop branches on material, distance and receding
+// (getForcesCollisionSingleGPU, Code/ImpactCode/impactAddTime.cu:245-377).
+// Here the same force is produced by three passes:
+//
+//   1. k_gravity  -- all pairs, branch free: sum_j (G mi mj) dp / r^3 with a
+//      hardware rsq, i-particles in registers (R per lane), j-particles read
+//      through wave-uniform scalar loads, float sums per j-tile folded into a
+//      Kahan-compensated accumulator, one j-chunk per wavefront, partial sums
+//      to a slot buffer.
+//   2. the contact grid -- a spatial hash of cells of size 1 (the element
+//      diameter) built by counting sort (k_grid_*).
+//   3. k_finish   -- per particle: sum of the j-chunk partials in fixed order
+//      plus, for every pair with r < 1 found through the grid, the
+//      reference's exact piecewise contact force minus what pass 1 added for
+//      that pair.
+//
+// Only pairs with r < 1 can differ from gravity (impactAddTime.cu:300,320,349),
+// so the sum equals the reference's force up to float rounding, and the
+// contact branch decisions are made with the reference's own arithmetic.
+#include "gi_device.h"
+
+namespace gi {
+
+// ---------------------------------------------------------------------------
+// Pass 1: all-pairs gravity (and the potential variant for the energy check)
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ void kahan_add(float& sum, float& comp, float v) {
+    float y = v - comp;
+    float t = sum + y;
+    comp = (t - sum) - y;
+    sum = t;
+}
+
+// R: i-particles per lane.  T: j per float tile between Kahan folds.
+// POT: accumulate sum_j 1/r (self excluded) instead of sum_j dp/r^3.
+template <int R, int T, bool POT>
+__global__ __launch_bounds__(256) void k_gravity(const float4* __restrict__ pos,
+                                                 const GravTask* __restrict__ tasks, int ntasks,
+                                                 const SimDesc* __restrict__ sims,
+                                                 float4* __restrict__ partial) {
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+    if (w >= ntasks) return;
+    const GravTask t = tasks[w];
+    const int start = sims[t.sim].start;
+    const int fe_end = sims[t.sim].fe_end;
+    const int end = sims[t.sim].end;
+    const int pbase = sims[t.sim].partial_base;
+    const float gFeFe = sims[t.sim].law.GMassFeFe;
+    const float gFeSi = sims[t.sim].law.GMassFeSi;
+
+    float xi[R], yi[R], zi[R];
+    float ax[R], ay[R], az[R], cx[R], cy[R], cz[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        int i = t.i0 + r * 64 + lane;
+        int ic = i < end ? i : end - 1;
+        float4 p = pos[ic];
+        xi[r] = p.x;
+        yi[r] = p.y;
+        zi[r] = p.z;
+        ax[r] = ay[r] = az[r] = 0.0f;
+        cx[r] = cy[r] = cz[r] = 0.0f;
+    }
+
+    for (int jb = t.j0; jb < t.j1; jb += T) {
+        const int je = min(jb + T, t.j1);
+        float tx[R], ty[R], tz[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) tx[r] = ty[r] = tz[r] = 0.0f;
+#pragma unroll 4
+        for (int j = jb; j < je; ++j) {
+            const float4 pj = pos[j];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                float dx = pj.x - xi[r];
+                float dy = pj.y - yi[r];
+                float dz = pj.z - zi[r];
+                if constexpr (!POT) {
+                    float s = fast_inv_r3(fast_r2(dx, dy, dz));
+                    tx[r] = __builtin_fmaf(s, dx, tx[r]);
+                    ty[r] = __builtin_fmaf(s, dy, ty[r]);
+                    tz[r] = __builtin_fmaf(s, dz, tz[r]);
+                } else {
+                    float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+                    float ri = r2 > 0.0f ? __builtin_amdgcn_rsqf(r2) : 0.0f;
+                    tx[r] += ri;
+                }
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            kahan_add(ax[r], cx[r], tx[r]);
+            if constexpr (!POT) {
+                kahan_add(ay[r], cy[r], ty[r]);
+                kahan_add(az[r], cz[r], tz[r]);
+            }
+        }
+    }
+
+    const int n = end - start;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        int i = t.i0 + r * 64 + lane;
+        if (i < end) {
+            const bool ife = i < fe_end;
+            const float g = t.jfe ? (ife ? gFeFe : gFeSi) : (ife ? gFeSi : 1.0f);
+            float4 o;
+            o.x = (ax[r] - cx[r]) * g;
+            o.y = (ay[r] - cy[r]) * g;
+            o.z = (az[r] - cz[r]) * g;
+            o.w = 0.0f;
+            partial[pbase + t.chunk * n + (i - start)] = o;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Pass 2: spatial hash of unit cells (counting sort, deterministic buckets)
+// ---------------------------------------------------------------------------
+
+__global__ void k_fill_int(int* __restrict__ a, int n, int v) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
+// sim_of: for each particle its simulation index (particle ranges are given
+// by sims[]); computed by a binary search over the sorted starts.
+__device__ __forceinline__ int find_sim(const SimDesc* __restrict__ sims, int nsims, int i) {
+    int lo = 0, hi = nsims - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (sims[mid].start <= i)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ void k_grid_count(const float4* __restrict__ pos, int ntot, const SimDesc* __restrict__ sims,
+                             int nsims, unsigned mask, int* __restrict__ counts,
+                             int* __restrict__ cellhash) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ntot) return;
+    int s = find_sim(sims, nsims, i);
+    float4 p = pos[i];
+    unsigned h = cell_hash(cell_coord(p.x), cell_coord(p.y), cell_coord(p.z), s) & mask;
+    cellhash[i] = (int)h;
+    atomicAdd(&counts[h], 1);
+}
+
+// Exclusive scan of counts[0..m) into starts[0..m], 1024 elements per block.
+__global__ __launch_bounds__(256) void k_scan_blocks(const int* __restrict__ in, int m,
+                                                     int* __restrict__ out,
+                                                     int* __restrict__ block_sums) {
+    __shared__ int sh[256];
+    const int base = blockIdx.x * 1024 + threadIdx.x * 4;
+    int v[4];
+    int local = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        v[k] = (base + k < m) ? in[base + k] : 0;
+        local += v[k];
+    }
+    sh[threadIdx.x] = local;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        int add = threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
+        __syncthreads();
+        sh[threadIdx.x] += add;
+        __syncthreads();
+    }
+    int run = sh[threadIdx.x] - local;  // exclusive prefix of this thread
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (base + k < m) out[base + k] = run;
+        run += v[k];
+    }
+    if (threadIdx.x == 255) block_sums[blockIdx.x] = sh[255];
+}
+
+// Exclusive scan of the block sums in one block (nb <= 1024 * 4).
+__global__ __launch_bounds__(1024) void k_scan_top(int* __restrict__ block_sums, int nb,
+                                                   int* __restrict__ total_out) {
+    __shared__ int sh[1024];
+    const int base = threadIdx.x * 4;
+    int v[4];
+    int local = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        v[k] = (base + k < nb) ? block_sums[base + k] : 0;
+        local += v[k];
+    }
+    sh[threadIdx.x] = local;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        int add = threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
+        __syncthreads();
+        sh[threadIdx.x] += add;
+        __syncthreads();
+    }
+    int run = sh[threadIdx.x] - local;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (base + k < nb) block_sums[base + k] = run;
+        run += v[k];
+    }
+    if (threadIdx.x == 1023) *total_out = sh[1023];
+}
+
+__global__ void k_scan_add(int* __restrict__ out, int m, const int* __restrict__ block_sums) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m) out[i] += block_sums[i >> 10];
+}
+
+// Scatter particle ids into their buckets; counts[] is consumed back to zero.
+__global__ void k_grid_scatter(int ntot, const int* __restrict__ cellhash,
+                               const int* __restrict__ starts, int* __restrict__ counts,
+                               int* __restrict__ sorted) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ntot) return;
+    int h = cellhash[i];
+    int slot = starts[h] + atomicSub(&counts[h], 1) - 1;
+    sorted[slot] = i;
+}
+
+// Sort each bucket by particle id so the contact sums run in a fixed order.
+__global__ void k_grid_sort_buckets(int m, const int* __restrict__ starts, int* __restrict__ sorted) {
+    int h = blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= m) return;
+    int s = starts[h], e = starts[h + 1];
+    for (int a = s + 1; a < e; ++a) {
+        int key = sorted[a];
+        int b = a - 1;
+        while (b >= s && sorted[b] > key) {
+            sorted[b + 1] = sorted[b];
+            --b;
+        }
+        sorted[b + 1] = key;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Pass 3: chunk sums + exact contact correction
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void k_finish(const float4* __restrict__ pos,
+                                                const float4* __restrict__ vel,
+                                                float4* __restrict__ force, int ntot,
+                                                const SimDesc* __restrict__ sims, int nsims,
+                                                const float4* __restrict__ partial,
+                                                const int* __restrict__ starts,
+                                                const int* __restrict__ sorted, unsigned mask) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ntot) return;
+    const int s = find_sim(sims, nsims, i);
+    const SimDesc& sd = sims[s];
+    if (!sd.active) return;
+    const int n = sd.end - sd.start;
+
+    float fx = 0.0f, fy = 0.0f, fz = 0.0f, kx = 0.0f, ky = 0.0f, kz = 0.0f;
+    for (int c = 0; c < sd.nchunk; ++c) {
+        float4 p = partial[sd.partial_base + c * n + (i - sd.start)];
+        kahan_add(fx, kx, p.x);
+        kahan_add(fy, ky, p.y);
+        kahan_add(fz, kz, p.z);
+    }
+    fx -= kx;
+    fy -= ky;
+    fz -= kz;
+
+    const float4 pi = pos[i];
+    const float4 vi = vel[i];
+    const int ccx = cell_coord(pi.x), ccy = cell_coord(pi.y), ccz = cell_coord(pi.z);
+    const bool ife = i < sd.fe_end;
+    float ex = 0.0f, ey = 0.0f, ez = 0.0f;
+    for (int oz = -1; oz <= 1; ++oz)
+        for (int oy = -1; oy <= 1; ++oy)
+            for (int ox = -1; ox <= 1; ++ox) {
+                const int qx = ccx + ox, qy = ccy + oy, qz = ccz + oz;
+                const unsigned h = cell_hash(qx, qy, qz, s) & mask;
+                const int b0 = starts[h], b1 = starts[h + 1];
+                for (int k = b0; k < b1; ++k) {
+                    const int j = sorted[k];
+                    if (j == i || j < sd.start || j >= sd.end) continue;
+                    const float4 pj = pos[j];
+                    if (cell_coord(pj.x) != qx || cell_coord(pj.y) != qy || cell_coord(pj.z) != qz)
+                        continue;  // another cell sharing the bucket
+                    const float dpx = pj.x - pi.x;
+                    const float dpy = pj.y - pi.y;
+                    const float dpz = pj.z - pi.z;
+                    const float r2 = dpx * dpx + dpy * dpy + dpz * dpz;
+                    const float r = sqrtf(r2);
+                    if (1.0f <= r) continue;
+                    const float invr = 1.0f / r;
+                    const int test = (int)ife + (int)(j < sd.fe_end);
+                    const float4 vj = vel[j];
+                    const float dvx = vj.x - vi.x, dvy = vj.y - vi.y, dvz = vj.z - vi.z;
+                    const float dot = dpx * dvx + dpy * dvy + dpz * dvz;
+                    const float fm = contact_pair_mag(sd.law, test, r, r2, dot);
+                    // What pass 1 added for this pair.
+                    const float g = gravity_product(sd.law, test);
+                    const float sf = g * fast_inv_r3(fast_r2(dpx, dpy, dpz));
+                    ex += fm * dpx * invr - sf * dpx;
+                    ey += fm * dpy * invr - sf * dpy;
+                    ez += fm * dpz * invr - sf * dpz;
+                }
+            }
+    force[i].x = fx + ex;
+    force[i].y = fy + ey;
+    force[i].z = fz + ez;
+}
+
+// ---------------------------------------------------------------------------
+// Move kernels
+// ---------------------------------------------------------------------------
+
+// moveBodiesCollisionSingleGPU (impactAddTime.cu:379-394), generalised to
+// packed simulations.  No contraction: v + f*t and x + v*dt round as written.
+__global__ void k_move(float4* __restrict__ pos, float4* __restrict__ vel,
+                       const float4* __restrict__ force, int ntot, const SimDesc* __restrict__ sims,
+                       int nsims) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ntot) return;
+    const SimDesc& sd = sims[nsims == 1 ? 0 : find_sim(sims, nsims, i)];
+    if (!sd.active) return;
+    const float temp = i < sd.fe_end ? sd.dt_over_mass_fe : sd.dt_over_mass_si;
+    float4 v = vel[i];
+    float4 p = pos[i];
+    const float4 f = force[i];
+    v.x += f.x * temp;
+    v.y += f.y * temp;
+    v.z += f.z * temp;
+    p.x += v.x * sd.dt;
+    p.y += v.y * sd.dt;
+    p.z += v.z * sd.dt;
+    vel[i] = v;
+    pos[i] = p;
+}
+
+// moveBodiesCollisionSingleGPU with its constants by value (the drop-in path).
+__global__ void k_move_collision(float4* __restrict__ pos, float4* __restrict__ vel,
+                                 const float4* __restrict__ force, int n, float dt,
+                                 float dt_over_mass_fe, float dt_over_mass_si, int nfe) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float temp = i < nfe ? dt_over_mass_fe : dt_over_mass_si;
+    float4 v = vel[i];
+    float4 p = pos[i];
+    const float4 f = force[i];
+    v.x += f.x * temp;
+    v.y += f.y * temp;
+    v.z += f.z * temp;
+    p.x += v.x * dt;
+    p.y += v.y * dt;
+    p.z += v.z * dt;
+    vel[i] = v;
+    pos[i] = p;
+}
+
+// moveBodiesDampedSeperate (generateTargetImpactor.cu:628-664).
+__global__ void k_move_damped_separate(float4* __restrict__ pos, float4* __restrict__ vel,
+                                       const float4* __restrict__ force, int n, float dt,
+                                       float dt_over_mass_fe, float dt_over_mass_si, int b1, int b2,
+                                       int b3, float damp1, float damp2) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float temp, damp;
+    if (b3 <= i) {
+        temp = dt_over_mass_si;
+        damp = damp2;
+    } else if (b2 <= i) {
+        temp = dt_over_mass_fe;
+        damp = damp2;
+    } else if (b1 <= i) {
+        temp = dt_over_mass_si;
+        damp = damp1;
+    } else {
+        temp = dt_over_mass_fe;
+        damp = damp1;
+    }
+    float4 v = vel[i];
+    float4 p = pos[i];
+    const float4 f = force[i];
+    v.x += (f.x - damp * v.x) * temp;
+    v.y += (f.y - damp * v.y) * temp;
+    v.z += (f.z - damp * v.z) * temp;
+    p.x += v.x * dt;
+    p.y += v.y * dt;
+    p.z += v.z * dt;
+    vel[i] = v;
+    pos[i] = p;
+}
+
+// ---------------------------------------------------------------------------
+// FP64 reductions: mass moments, momentum, angular momentum, energy
+// ---------------------------------------------------------------------------
+
+// Per-particle potential: -(1/2) * (sum of the chunk partials of 1/r scaled
+// by G mi mj) plus, for pairs with r < 1, half the difference between the
+// exact contact potential and the gravity term pass 1 added.  FP64 output.
+__global__ __launch_bounds__(256) void k_finish_potential(const float4* __restrict__ pos, int ntot,
+                                                          const SimDesc* __restrict__ sims,
+                                                          int nsims,
+                                                          const float4* __restrict__ partial,
+                                                          const int* __restrict__ starts,
+                                                          const int* __restrict__ sorted,
+                                                          unsigned mask, double* __restrict__ pe) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ntot) return;
+    const int s = find_sim(sims, nsims, i);
+    const SimDesc& sd = sims[s];
+    const int n = sd.end - sd.start;
+    double phi = 0.0;
+    for (int c = 0; c < sd.nchunk; ++c) phi += (double)partial[sd.partial_base + c * n + (i - sd.start)].x;
+    double u = -phi;
+    const float4 pi = pos[i];
+    const int ccx = cell_coord(pi.x), ccy = cell_coord(pi.y), ccz = cell_coord(pi.z);
+    const bool ife = i < sd.fe_end;
+    for (int oz = -1; oz <= 1; ++oz)
+        for (int oy = -1; oy <= 1; ++oy)
+            for (int ox = -1; ox <= 1; ++ox) {
+                const int qx = ccx + ox, qy = ccy + oy, qz = ccz + oz;
+                const unsigned h = cell_hash(qx, qy, qz, s) & mask;
+                for (int k = starts[h]; k < starts[h + 1]; ++k) {
+                    const int j = sorted[k];
+                    if (j == i || j < sd.start || j >= sd.end) continue;
+                    const float4 pj = pos[j];
+                    if (cell_coord(pj.x) != qx || cell_coord(pj.y) != qy || cell_coord(pj.z) != qz)
+                        continue;
+                    const double dx = (double)pj.x - pi.x, dy = (double)pj.y - pi.y,
+                                 dz = (double)pj.z - pi.z;
+                    const double r = sqrt(dx * dx + dy * dy + dz * dz);
+                    if (r >= 1.0) continue;
+                    const int test = (int)ife + (int)(j < sd.fe_end);
+                    const double G = gravity_product(sd.law, test);
+                    const double K = approach_k(sd.law, test);
+                    const double exact = G * (r - 2.0) + (K / 3.0) * (r - 1.0) * (r - 1.0) * (r + 2.0);
+                    const float fx = pj.x - pi.x, fy = pj.y - pi.y, fz = pj.z - pi.z;
+                    const float r2f = __builtin_fmaf(fz, fz, __builtin_fmaf(fy, fy, fx * fx));
+                    const double fast = -G * (double)__builtin_amdgcn_rsqf(r2f);
+                    u += exact - fast;
+                }
+            }
+    pe[i] = 0.5 * u;
+}
+
+// Block partial sums of the 14 FP64 moments of the particles of one
+// simulation: M, M x (3), M v (3), sum m x cross v (3), kinetic, potential,
+// sum m |x|^2 (unused by callers, kept for the rigid-body test), count.
+constexpr int kMoments = 14;
+
+__global__ __launch_bounds__(256) void k_moments(const float4* __restrict__ pos,
+                                                 const float4* __restrict__ vel,
+                                                 const double* __restrict__ pe,
+                                                 const int* __restrict__ block_sim,
+                                                 const int* __restrict__ block_first,
+                                                 const SimDesc* __restrict__ sims,
+                                                 double mass_fe_override, double mass_si_override,
+                                                 double* __restrict__ out) {
+    __shared__ double sh[kMoments][256];
+    const int s = block_sim[blockIdx.x];
+    const int i = block_first[blockIdx.x] + threadIdx.x;
+    const SimDesc& sd = sims[s];
+    double v[kMoments];
+#pragma unroll
+    for (int k = 0; k < kMoments; ++k) v[k] = 0.0;
+    if (i < sd.end) {
+        double m;
+        if (mass_fe_override > 0.0)
+            m = i < sd.fe_end ? mass_fe_override : mass_si_override;
+        else
+            m = i < sd.fe_end ? (double)sd.law.GMassFeSi : 1.0;
+        const float4 p = pos[i];
+        const float4 u = vel[i];
+        const double x = p.x, y = p.y, z = p.z, a = u.x, b = u.y, c = u.z;
+        v[0] = m;
+        v[1] = m * x;
+        v[2] = m * y;
+        v[3] = m * z;
+        v[4] = m * a;
+        v[5] = m * b;
+        v[6] = m * c;
+        v[7] = m * (y * c - z * b);
+        v[8] = m * (z * a - x * c);
+        v[9] = m * (x * b - y * a);
+        v[10] = 0.5 * m * (a * a + b * b + c * c);
+        v[11] = pe ? pe[i] : 0.0;
+        v[12] = m * (x * x + y * y + z * z);
+        v[13] = 1.0;
+    }
+#pragma unroll
+    for (int k = 0; k < kMoments; ++k) sh[k][threadIdx.x] = v[k];
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < off) {
+#pragma unroll
+            for (int k = 0; k < kMoments; ++k) sh[k][threadIdx.x] += sh[k][threadIdx.x + off];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < kMoments) out[blockIdx.x * kMoments + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+// Sum the block moments of each simulation in block order (one thread per
+// simulation) and, for the recenter, turn them into the shift to apply.
+__global__ void k_moments_final(const double* __restrict__ blocks, const int* __restrict__ sim_first_block,
+                                int nsims, double* __restrict__ sums) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nsims) return;
+    double acc[kMoments];
+#pragma unroll
+    for (int k = 0; k < kMoments; ++k) acc[k] = 0.0;
+    for (int b = sim_first_block[s]; b < sim_first_block[s + 1]; ++b)
+#pragma unroll
+        for (int k = 0; k < kMoments; ++k) acc[k] += blocks[b * kMoments + k];
+#pragma unroll
+    for (int k = 0; k < kMoments; ++k) sums[s * kMoments + k] = acc[k];
+}
+
+// Recenter shift: positions += target_com - com, velocities += target_vel - vbar.
+__global__ void k_recenter_apply(float4* __restrict__ pos, float4* __restrict__ vel, int n,
+                                 const double* __restrict__ sums, const double* __restrict__ target6) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double M = sums[0];
+    const float sx = (float)(target6[0] - sums[1] / M);
+    const float sy = (float)(target6[1] - sums[2] / M);
+    const float sz = (float)(target6[2] - sums[3] / M);
+    const float ux = (float)(target6[3] - sums[4] / M);
+    const float uy = (float)(target6[4] - sums[5] / M);
+    const float uz = (float)(target6[5] - sums[6] / M);
+    float4 p = pos[i];
+    float4 v = vel[i];
+    p.x += sx;
+    p.y += sy;
+    p.z += sz;
+    v.x += ux;
+    v.y += uy;
+    v.z += uz;
+    pos[i] = p;
+    vel[i] = v;
+}
+
+// Explicit instantiations used by the host code.
+template __global__ void k_gravity<4, 128, false>(const float4*, const GravTask*, int, const SimDesc*, float4*);
+template __global__ void k_gravity<8, 128, false>(const float4*, const GravTask*, int, const SimDesc*, float4*);
+template __global__ void k_gravity<2, 128, false>(const float4*, const GravTask*, int, const SimDesc*, float4*);
+template __global__ void k_gravity<4, 128, true>(const float4*, const GravTask*, int, const SimDesc*, float4*);
+
+}  // namespace gi

@@ -1,0 +1,428 @@
+"""ctypes bindings of libgiant_impact.so (the C ABI in include/giant_impact.h).
+
+The library is the product: HIP kernels for gfx950 behind a C ABI.  This
+module only marshals arguments; there is no Python or CPU fallback.  Loading
+fails loudly when the shared library has not been built.
+
+Device memory is handled through the library's own gi_malloc / gi_copy_*
+(``DeviceArray``), so no other GPU framework is needed.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+PKG_DIR = Path(__file__).resolve().parent.parent
+LIB_PATH = PKG_DIR / "lib" / "libgiant_impact.so"
+HEADER_PATH = PKG_DIR.parent / "include" / "giant_impact.h"
+
+
+class GIError(RuntimeError):
+    def __init__(self, func: str, code: int, message: str):
+        super().__init__(f"{func} failed with code {code}: {message}")
+        self.code = code
+
+
+# ---------------------------------------------------------------------------
+# ABI structures (layout-identical to the reference's kernel structs)
+# ---------------------------------------------------------------------------
+
+_LAW_FIELDS = [
+    ("GMassFeFe", C.c_float),
+    ("GMassFeSi", C.c_float),
+    ("KFeFe", C.c_float),
+    ("KSiSi", C.c_float),
+    ("KFeSi", C.c_float),
+    ("KRFeFe", C.c_float),
+    ("KRSiSi", C.c_float),
+    ("KRFeSi", C.c_float),
+    ("KRMix", C.c_float),
+    ("ShellBreakFe", C.c_float),
+    ("ShellBreakSi", C.c_float),
+    ("ShellBreakFeSi1", C.c_float),
+    ("ShellBreakFeSi2", C.c_float),
+]
+
+
+class ForceCollisionConstants(C.Structure):
+    """forceCollisionKernalConstantsStruct (Code/ImpactCode/impactAddTime.cu:4-20)."""
+
+    _fields_ = _LAW_FIELDS + [("NFe", C.c_int)]
+
+
+class MoveCollisionConstants(C.Structure):
+    """moveCollisionKernalConstantsStruct (Code/ImpactCode/impactAddTime.cu:22-28)."""
+
+    _fields_ = [("Dt", C.c_float), ("DtOverMassFe", C.c_float), ("DtOverMassSi", C.c_float), ("NFe", C.c_int)]
+
+
+class ForceSeparateConstants(C.Structure):
+    """forceSeperateKernalConstantsStruct (generateTargetImpactor.cu:3-21)."""
+
+    _fields_ = _LAW_FIELDS + [("boarder1", C.c_int), ("boarder2", C.c_int), ("boarder3", C.c_int)]
+
+
+class MoveSeparateConstants(C.Structure):
+    """moveSeperateKernalConstantsStruct (generateTargetImpactor.cu:23-31)."""
+
+    _fields_ = [
+        ("Dt", C.c_float),
+        ("DtOverMassFe", C.c_float),
+        ("DtOverMassSi", C.c_float),
+        ("boarder1", C.c_int),
+        ("boarder2", C.c_int),
+        ("boarder3", C.c_int),
+    ]
+
+
+class GeneratorSetup(C.Structure):
+    _fields_ = [
+        ("initial_spin1", C.c_float * 4),
+        ("initial_spin2", C.c_float * 4),
+        ("fraction_earth_mass_body1", C.c_double),
+        ("fraction_earth_mass_body2", C.c_double),
+        ("fraction_fe_body1", C.c_double),
+        ("fraction_si_body1", C.c_double),
+        ("fraction_fe_body2", C.c_double),
+        ("fraction_si_body2", C.c_double),
+        ("damp_rate_body1", C.c_float),
+        ("damp_rate_body2", C.c_float),
+        ("total_number_of_elements", C.c_int),
+        ("damp_time", C.c_double),
+        ("damp_rest_time", C.c_double),
+        ("spin_rest_time", C.c_double),
+        ("dt", C.c_double),
+        ("density_fe", C.c_double),
+        ("density_si", C.c_double),
+        ("k_fe", C.c_double),
+        ("k_si", C.c_double),
+        ("kr_fe", C.c_double),
+        ("kr_si", C.c_double),
+        ("sd_fe", C.c_double),
+        ("sd_si", C.c_double),
+        ("draw_rate", C.c_int),
+    ]
+
+
+class Units(C.Structure):
+    _fields_ = [
+        ("nfe", C.c_int),
+        ("nsi", C.c_int),
+        ("nfe1", C.c_int),
+        ("nsi1", C.c_int),
+        ("nfe2", C.c_int),
+        ("nsi2", C.c_int),
+        ("unit_length", C.c_double),
+        ("unit_mass", C.c_double),
+        ("unit_time", C.c_double),
+        ("mass_fe", C.c_double),
+        ("mass_si", C.c_double),
+        ("diameter", C.c_double),
+        ("gravity", C.c_double),
+        ("k_fe", C.c_double),
+        ("k_si", C.c_double),
+        ("kr_fe", C.c_double),
+        ("kr_si", C.c_double),
+        ("sd_fe", C.c_double),
+        ("sd_si", C.c_double),
+        ("mass_of_body1", C.c_double),
+        ("mass_of_body2", C.c_double),
+        ("damp_time", C.c_double),
+        ("damp_rest_time", C.c_double),
+        ("spin_rest_time", C.c_double),
+        ("spin1_w", C.c_float),
+        ("spin2_w", C.c_float),
+    ]
+
+
+class ImpactConfig(C.Structure):
+    _fields_ = [
+        ("n_target", C.c_int),
+        ("n_impactor", C.c_int),
+        ("fe_fraction", C.c_double),
+        ("jitter", C.c_double),
+        ("seed", C.c_ulonglong),
+        ("angle_deg", C.c_double),
+        ("speed_vesc", C.c_double),
+        ("target_spin", C.c_double),
+        ("impactor_spin", C.c_double),
+        ("gap_in_target_radii", C.c_double),
+    ]
+
+
+class ImpactInfo(C.Structure):
+    _fields_ = [
+        ("n", C.c_int),
+        ("nfe", C.c_int),
+        ("nfe1", C.c_int),
+        ("nfe2", C.c_int),
+        ("nsi1", C.c_int),
+        ("nsi2", C.c_int),
+        ("mass_fe", C.c_double),
+        ("mass_si", C.c_double),
+        ("radius_target", C.c_double),
+        ("radius_impactor", C.c_double),
+        ("mass_target", C.c_double),
+        ("mass_impactor", C.c_double),
+        ("v_escape", C.c_double),
+        ("v_relative", C.c_double),
+        ("impact_parameter", C.c_double),
+        ("target_omega", C.c_double),
+        ("impactor_omega", C.c_double),
+        ("com", C.c_double * 3),
+        ("vel", C.c_double * 3),
+    ]
+
+
+# ---------------------------------------------------------------------------
+# Library loading
+# ---------------------------------------------------------------------------
+
+_lib = None
+
+
+def lib_path() -> Path:
+    return LIB_PATH
+
+
+def load() -> C.CDLL:
+    """Load libgiant_impact.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it first (python -c 'import __graft_entry__ as g; g.build()' "
+            f"or make -C {PKG_DIR})"
+        )
+    lib = C.CDLL(str(LIB_PATH))
+    vp, i, sz = C.c_void_p, C.c_int, C.c_size_t
+    P = C.POINTER
+    sig = {
+        "gi_abi_version": (i, []),
+        "gi_last_error_string": (C.c_char_p, []),
+        "gi_device_count": (i, []),
+        "gi_set_device": (i, [i]),
+        "gi_check_block_multiple": (i, [i]),
+        "gi_malloc": (i, [P(vp), sz]),
+        "gi_free": (i, [vp]),
+        "gi_copy_h2d": (i, [vp, vp, sz]),
+        "gi_copy_d2h": (i, [vp, vp, sz]),
+        "gi_copy_d2d": (i, [vp, vp, sz, vp]),
+        "gi_stream_create": (i, [P(vp)]),
+        "gi_stream_destroy": (i, [vp]),
+        "gi_stream_sync": (i, [vp]),
+        "gi_device_sync": (i, []),
+        "gi_mem_info": (i, [P(sz), P(sz)]),
+        "gi_force_collision": (i, [vp, vp, vp, i, P(ForceCollisionConstants), vp]),
+        "gi_move_collision": (i, [vp, vp, vp, i, P(MoveCollisionConstants), vp]),
+        "gi_force_separate": (i, [vp, vp, vp, i, P(ForceSeparateConstants), vp]),
+        "gi_move_damped_separate": (
+            i,
+            [vp, vp, vp, i, P(MoveSeparateConstants), C.c_float, C.c_float, vp],
+        ),
+        "gi_set_kernel_timing": (i, [i]),
+        "gi_kernel_timing": (i, [C.c_char_p, P(C.c_longlong), P(C.c_double)]),
+        "gi_reset_kernel_timing": (i, []),
+        "gi_generator_setup_defaults": (None, [P(GeneratorSetup), i]),
+        "gi_generator_setup_read": (i, [C.c_char_p, P(GeneratorSetup)]),
+        "gi_units_compute": (i, [P(GeneratorSetup), P(Units)]),
+        "gi_separate_constants": (None, [P(Units), C.c_double, P(ForceSeparateConstants), P(MoveSeparateConstants)]),
+        "gi_impact_constants": (None, [P(Units), C.c_double, P(ForceCollisionConstants), P(MoveCollisionConstants)]),
+        "gi_parameters_write": (i, [C.c_char_p, P(Units), i]),
+        "gi_parameters_read": (i, [C.c_char_p, P(Units), P(i)]),
+        "gi_make_impact": (
+            i,
+            [P(ImpactConfig), vp, vp, P(ForceCollisionConstants), P(MoveCollisionConstants), P(ImpactInfo)],
+        ),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _optional = {
+        "gi_center_of_mass_velocity": (i, [vp, vp, i, i, C.c_double, C.c_double, P(C.c_double), vp]),
+        "gi_recenter": (i, [vp, vp, i, i, C.c_double, C.c_double, P(C.c_double), P(C.c_double), vp]),
+        "gi_conserved_quantities": (i, [vp, vp, i, P(ForceCollisionConstants), P(C.c_double), vp]),
+        "gi_impact_steps": (
+            i,
+            [vp, vp, vp, i, P(ForceCollisionConstants), P(MoveCollisionConstants), i, i, P(C.c_double), P(C.c_double), vp],
+        ),
+        "gi_ensemble_create": (
+            i,
+            [P(vp), i, P(i), P(ForceCollisionConstants), P(MoveCollisionConstants), vp, vp, vp],
+        ),
+        "gi_ensemble_destroy": (i, [vp]),
+        "gi_ensemble_steps": (i, [vp, i]),
+        "gi_ensemble_set_active": (i, [vp, P(i)]),
+        "gi_ensemble_conserved": (i, [vp, P(C.c_double)]),
+        "gi_ensemble_work": (i, [vp, P(C.c_longlong), P(C.c_double)]),
+        "gi_set_tiling": (i, [i, C.c_longlong, C.c_longlong, C.c_longlong]),
+    }
+    for name, (res, args) in _optional.items():
+        if hasattr(lib, name):
+            f = getattr(lib, name)
+            f.restype = res
+            f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(func: str, rc: int) -> None:
+    if rc != 0:
+        msg = load().gi_last_error_string()
+        raise GIError(func, rc, msg.decode() if msg else "")
+
+
+def call(name: str, *args):
+    rc = getattr(load(), name)(*args)
+    check(name, rc)
+    return rc
+
+
+def device_count() -> int:
+    return load().gi_device_count()
+
+
+# ---------------------------------------------------------------------------
+# Device arrays of float4 records
+# ---------------------------------------------------------------------------
+
+
+class DeviceArray:
+    """n float4 records in device memory (gi_malloc), freed on close()/GC."""
+
+    def __init__(self, n: int, init: np.ndarray | None = None):
+        self.n = int(n)
+        self.nbytes = max(16 * self.n, 16)
+        self.ptr = C.c_void_p()
+        call("gi_malloc", C.byref(self.ptr), self.nbytes)
+        if init is not None:
+            self.upload(init)
+
+    @property
+    def value(self) -> int:
+        return self.ptr.value
+
+    def upload(self, a: np.ndarray) -> None:
+        a = np.ascontiguousarray(a, dtype=np.float32).reshape(-1, 4)
+        assert a.shape[0] == self.n
+        if self.n:
+            call("gi_copy_h2d", self.ptr, a.ctypes.data_as(C.c_void_p), 16 * self.n)
+
+    def download(self) -> np.ndarray:
+        out = np.empty((self.n, 4), dtype=np.float32)
+        if self.n:
+            call("gi_copy_d2h", out.ctypes.data_as(C.c_void_p), self.ptr, 16 * self.n)
+        return out
+
+    def close(self) -> None:
+        if self.ptr is not None and self.ptr.value:
+            load().gi_free(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------------------
+# Host-side helpers
+# ---------------------------------------------------------------------------
+
+
+def generator_setup_defaults(n: int) -> GeneratorSetup:
+    s = GeneratorSetup()
+    load().gi_generator_setup_defaults(C.byref(s), n)
+    return s
+
+
+def units(setup: GeneratorSetup) -> Units:
+    u = Units()
+    call("gi_units_compute", C.byref(setup), C.byref(u))
+    return u
+
+
+def impact_constants(u: Units, dt: float = 0.002):
+    fc, mc = ForceCollisionConstants(), MoveCollisionConstants()
+    load().gi_impact_constants(C.byref(u), dt, C.byref(fc), C.byref(mc))
+    return fc, mc
+
+
+def separate_constants(u: Units, dt: float = 0.002):
+    fc, mc = ForceSeparateConstants(), MoveSeparateConstants()
+    load().gi_separate_constants(C.byref(u), dt, C.byref(fc), C.byref(mc))
+    return fc, mc
+
+
+def make_impact(
+    n_target: int,
+    n_impactor: int,
+    *,
+    fe_fraction: float = 0.3,
+    jitter: float = 0.1,
+    seed: int = 20261202,
+    angle_deg: float = 45.0,
+    speed_vesc: float = 1.2,
+    target_spin: float = 0.0,
+    impactor_spin: float = 0.0,
+    gap_in_target_radii: float = 1.0,
+):
+    """Seeded synthetic impact in the reference impact layout [Fe1, Fe2, Si1, Si2].
+
+    Returns (pos, vel, fc, mc, info) with pos/vel float32 arrays of shape (n, 4).
+    """
+    cfg = ImpactConfig(
+        n_target, n_impactor, fe_fraction, jitter, seed, angle_deg, speed_vesc, target_spin, impactor_spin,
+        gap_in_target_radii,
+    )
+    n = n_target + n_impactor
+    pos = np.zeros((n, 4), dtype=np.float32)
+    vel = np.zeros((n, 4), dtype=np.float32)
+    fc, mc, info = ForceCollisionConstants(), MoveCollisionConstants(), ImpactInfo()
+    call(
+        "gi_make_impact",
+        C.byref(cfg),
+        pos.ctypes.data_as(C.c_void_p),
+        vel.ctypes.data_as(C.c_void_p),
+        C.byref(fc),
+        C.byref(mc),
+        C.byref(info),
+    )
+    return pos, vel, fc, mc, info
+
+
+# ---------------------------------------------------------------------------
+# The four step-loop entry points (device pointers)
+# ---------------------------------------------------------------------------
+
+
+def _p(x) -> C.c_void_p:
+    if isinstance(x, DeviceArray):
+        return x.ptr
+    return C.c_void_p(int(x)) if x is not None else C.c_void_p()
+
+
+def force_collision(pos, vel, force, n: int, fc: ForceCollisionConstants, stream=None) -> None:
+    call("gi_force_collision", _p(pos), _p(vel), _p(force), int(n), C.byref(fc), _p(stream))
+
+
+def move_collision(pos, vel, force, n: int, mc: MoveCollisionConstants, stream=None) -> None:
+    call("gi_move_collision", _p(pos), _p(vel), _p(force), int(n), C.byref(mc), _p(stream))
+
+
+def force_separate(pos, vel, force, n: int, fc: ForceSeparateConstants, stream=None) -> None:
+    call("gi_force_separate", _p(pos), _p(vel), _p(force), int(n), C.byref(fc), _p(stream))
+
+
+def move_damped_separate(pos, vel, force, n: int, mc: MoveSeparateConstants, d1: float, d2: float, stream=None):
+    call("gi_move_damped_separate", _p(pos), _p(vel), _p(force), int(n), C.byref(mc), float(d1), float(d2), _p(stream))
+
+
+def sync() -> None:
+    call("gi_device_sync")

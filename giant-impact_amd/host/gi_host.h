@@ -1,0 +1,104 @@
+/*
+ * gi_host.h -- host side of the Giant-Impact step loop: units, constants and
+ * seeded synthetic bodies.  Part of libgiant_impact.so (C ABI).
+ */
+#ifndef GI_HOST_H
+#define GI_HOST_H
+
+#include "../../include/giant_impact.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Setup of the generator (SetupTemplates/setupGeneratingTargetImpactor:1-67),
+ * the values read by readTargetImpactorParameters
+ * (Code/GenerateTargetImpactorCode/generateTargetImpactor.cu:169-248), in
+ * file order. */
+typedef struct gi_generator_setup {
+    float initial_spin1[4]; /* direction xyz, revolutions per hour */
+    float initial_spin2[4];
+    double fraction_earth_mass_body1, fraction_earth_mass_body2;
+    double fraction_fe_body1, fraction_si_body1, fraction_fe_body2, fraction_si_body2;
+    float damp_rate_body1, damp_rate_body2;
+    int total_number_of_elements;
+    double damp_time, damp_rest_time, spin_rest_time; /* hours */
+    double dt;
+    double density_fe, density_si; /* kg km^-3 */
+    double k_fe, k_si;             /* kg km^-1 s^-2 */
+    double kr_fe, kr_si, sd_fe, sd_si;
+    int draw_rate;
+} gi_generator_setup;
+
+/* The unit system and counts of setTargetImpactorParameters
+ * (generateTargetImpactor.cu:289-403), before the %f round trip. */
+typedef struct gi_units {
+    int nfe, nsi, nfe1, nsi1, nfe2, nsi2;
+    double unit_length, unit_mass, unit_time; /* km, kg, s */
+    double mass_fe, mass_si, diameter, gravity;
+    double k_fe, k_si; /* simulation units */
+    double kr_fe, kr_si, sd_fe, sd_si;
+    double mass_of_body1, mass_of_body2;
+    double damp_time, damp_rest_time, spin_rest_time; /* simulation units */
+    float spin1_w, spin2_w;                           /* revolutions per time unit */
+} gi_units;
+
+/* The template defaults of SetupTemplates/setupGeneratingTargetImpactor with
+ * TotalNumberOfElements = n. */
+void gi_generator_setup_defaults(gi_generator_setup *s, int n);
+/* Parse a setupGeneratingTargetImpactor file the way the reference does:
+ * the value after each '=' in order, key names ignored. */
+int gi_generator_setup_read(const char *path, gi_generator_setup *s);
+/* setTargetImpactorParameters. Returns GI_E_INVALID_ARG when the Fe and Si
+ * fractions of a body do not add to 1 (generateTargetImpactor.cu:295-304). */
+int gi_units_compute(const gi_generator_setup *s, gi_units *u);
+/* loadKernalConstantStructures of the generator (generateTargetImpactor.cu:406-451). */
+void gi_separate_constants(const gi_units *u, double dt, gi_force_separate_constants *fc,
+                           gi_move_separate_constants *mc);
+/* loadKernalConstantStructures of the impact (impactAddTime.cu:128-170), from
+ * the values after the %f round trip of targetImpactorParameters. */
+void gi_impact_constants(const gi_units *u, double dt, gi_force_collision_constants *fc,
+                         gi_move_collision_constants *mc);
+/* Write / read targetImpactorParameters (generateTargetImpactorBasicFunctions.h:688-711,
+ * impactBasicFunctions.h:31-107). */
+int gi_parameters_write(const char *path, const gi_units *u, int total_number_of_elements);
+int gi_parameters_read(const char *path, gi_units *u, int *total_number_of_elements);
+
+/* Seeded synthetic impact: two jittered simple-cubic-lattice bodies. */
+typedef struct gi_impact_config {
+    int n_target, n_impactor;
+    double fe_fraction;       /* iron fraction by count, innermost */
+    double jitter;            /* uniform per-coordinate jitter, in diameters */
+    unsigned long long seed;  /* SplitMix64 seed */
+    double angle_deg;         /* impact angle (0 = head on) */
+    double speed_vesc;        /* relative speed in units of the mutual escape speed */
+    double target_spin;       /* target spin in units of its breakup rate */
+    double impactor_spin;     /* impactor spin in units of its breakup rate */
+    double gap_in_target_radii; /* initial surface gap (default 1.0) */
+} gi_impact_config;
+
+typedef struct gi_impact_info {
+    int n, nfe, nfe1, nfe2, nsi1, nsi2;
+    double mass_fe, mass_si;
+    double radius_target, radius_impactor;
+    double mass_target, mass_impactor;
+    double v_escape, v_relative, impact_parameter;
+    double target_omega, impactor_omega;
+    double com[3], vel[3];
+} gi_impact_info;
+
+/* Fill pos/vel (host arrays of n_target + n_impactor records) in the impact
+ * layout [Fe1, Fe2, Si1, Si2] (impactInitialization.cu:326-371) with .w =
+ * body/material tag of the generator (0 Fe1, 1 Si1, 2 Fe2, 3 Si2) in pos and
+ * the element mass in vel; the centre of mass is at rest at the origin.
+ * Constants come from the template defaults at n = n_target + n_impactor.
+ * Any of fc, mc, info may be NULL. */
+int gi_make_impact(const gi_impact_config *cfg, gi_float4 *pos, gi_float4 *vel,
+                   gi_force_collision_constants *fc, gi_move_collision_constants *mc,
+                   gi_impact_info *info);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,252 @@
+/*
+ * giant_impact.h -- C ABI of the MI355X (gfx950) step loop for Giant-Impact.
+ *
+ * The four kernels of the reference's step loop are replaced one for one:
+ *
+ *   getForcesCollisionSingleGPU   Code/ImpactCode/impactAddTime.cu:245-377   -> gi_force_collision
+ *   moveBodiesCollisionSingleGPU  Code/ImpactCode/impactAddTime.cu:379-394   -> gi_move_collision
+ *   getForcesSeperate             Code/GenerateTargetImpactorCode/generateTargetImpactor.cu:486-626
+ *                                                                            -> gi_force_separate
+ *   moveBodiesDampedSeperate      Code/GenerateTargetImpactorCode/generateTargetImpactor.cu:628-664
+ *                                                                            -> gi_move_damped_separate
+ *
+ * Every pointer argument named pos / vel / force is a DEVICE pointer to n
+ * float4 records (16 bytes each, .x .y .z used, .w passed through untouched),
+ * exactly the Pos_DEV0 / Vel_DEV0 / Force_DEV0 arrays of the reference
+ * (Code/CommonCode/commonSetup.h:38-39).  Material is given by index, as in
+ * the reference: in the impact layout [Fe1, Fe2, Si1, Si2] particle id is iron
+ * iff id < NFe; in the generation layout [Fe1, Si1, Fe2, Si2] the borders
+ * boarder1..3 split the bodies and materials.
+ *
+ * Unlike the reference kernels (launched <<<N/256, 256>>> with no bounds
+ * check, allocateMemoryAndSetupGPU at impactAddTime.cu:194-198 rejects
+ * N % 256 != 0) these entry points accept any n >= 0.  gi_check_block_multiple
+ * restates the reference's host-side N % 256 check for callers that want it.
+ *
+ * Return value of every int function: 0 on success, a positive hipError_t
+ * code when the HIP runtime failed, or a negative GI_E_* code for an invalid
+ * argument.  gi_last_error_string() describes the most recent failure of the
+ * calling thread.  (The reference prints and calls exit(0) instead,
+ * Code/CommonCode/commonFunctions.h:12-22.)
+ *
+ * Streams: a gi_stream_t is a hipStream_t; NULL means the default stream.
+ * Work is enqueued asynchronously in stream order.  Scratch buffers are owned
+ * by the library, one set per device, grown on demand and reused; calls that
+ * overlap on one device from several host threads or streams are not
+ * supported (the reference is single-threaded, one default stream).
+ */
+#ifndef GIANT_IMPACT_H
+#define GIANT_IMPACT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GI_ABI_VERSION 1
+
+/* 16-byte particle record, layout-identical to HIP/CUDA float4. */
+typedef struct gi_float4 {
+    float x, y, z, w;
+} gi_float4;
+
+typedef void *gi_stream_t;
+
+/* Negative error codes (positive codes are hipError_t values). */
+#define GI_OK 0
+#define GI_E_INVALID_ARG (-1)
+#define GI_E_NOT_MULTIPLE_OF_BLOCK (-2)
+#define GI_E_NO_DEVICE (-3)
+#define GI_E_TOO_LARGE (-4)
+#define GI_E_IO (-5)
+#define GI_E_PARSE (-6)
+#define GI_E_STATE (-7)
+
+/* ---- kernel constant structures (byte-identical to the reference) ------ */
+
+/* forceCollisionKernalConstantsStruct, Code/ImpactCode/impactAddTime.cu:4-20 (56 bytes). */
+typedef struct gi_force_collision_constants {
+    float GMassFeFe;
+    float GMassFeSi;
+    float KFeFe;
+    float KSiSi;
+    float KFeSi;
+    float KRFeFe;
+    float KRSiSi;
+    float KRFeSi;
+    float KRMix;
+    float ShellBreakFe;
+    float ShellBreakSi;
+    float ShellBreakFeSi1;
+    float ShellBreakFeSi2;
+    int NFe;
+} gi_force_collision_constants;
+
+/* moveCollisionKernalConstantsStruct, Code/ImpactCode/impactAddTime.cu:22-28 (16 bytes). */
+typedef struct gi_move_collision_constants {
+    float Dt;
+    float DtOverMassFe;
+    float DtOverMassSi;
+    int NFe;
+} gi_move_collision_constants;
+
+/* forceSeperateKernalConstantsStruct,
+ * Code/GenerateTargetImpactorCode/generateTargetImpactor.cu:3-21 (64 bytes). */
+typedef struct gi_force_separate_constants {
+    float GMassFeFe;
+    float GMassFeSi;
+    float KFeFe;
+    float KSiSi;
+    float KFeSi;
+    float KRFeFe;
+    float KRSiSi;
+    float KRFeSi;
+    float KRMix;
+    float ShellBreakFe;
+    float ShellBreakSi;
+    float ShellBreakFeSi1;
+    float ShellBreakFeSi2;
+    int boarder1; /* NFe1:              end of body-1 iron     */
+    int boarder2; /* NFe1+NSi1:         end of body 1          */
+    int boarder3; /* NFe1+NSi1+NFe2:    end of body-2 iron     */
+} gi_force_separate_constants;
+
+/* moveSeperateKernalConstantsStruct,
+ * Code/GenerateTargetImpactorCode/generateTargetImpactor.cu:23-31 (24 bytes). */
+typedef struct gi_move_separate_constants {
+    float Dt;
+    float DtOverMassFe;
+    float DtOverMassSi;
+    int boarder1;
+    int boarder2;
+    int boarder3;
+} gi_move_separate_constants;
+
+/* ---- library / device -------------------------------------------------- */
+
+int gi_abi_version(void);
+const char *gi_last_error_string(void);
+/* Number of visible HIP devices (0 when none); never fails. */
+int gi_device_count(void);
+/* Select the device used by subsequent calls from this thread. */
+int gi_set_device(int device);
+/* Restates the check of allocateMemoryAndSetupGPU (impactAddTime.cu:193-198):
+ * returns GI_E_NOT_MULTIPLE_OF_BLOCK when n % 256 != 0, else 0. */
+int gi_check_block_multiple(int n);
+
+/* Device memory and copies (copyPosVelToGPU / copyPosVelFromGPU,
+ * Code/CommonCode/commonFunctions.h:24-38).  gi_copy_* are blocking. */
+int gi_malloc(void **dptr, size_t bytes);
+int gi_free(void *dptr);
+int gi_copy_h2d(void *dst, const void *src, size_t bytes);
+int gi_copy_d2h(void *dst, const void *src, size_t bytes);
+int gi_copy_d2d(void *dst, const void *src, size_t bytes, gi_stream_t stream);
+int gi_stream_create(gi_stream_t *stream);
+int gi_stream_destroy(gi_stream_t stream);
+int gi_stream_sync(gi_stream_t stream);
+int gi_device_sync(void);
+/* Free and total device memory in bytes (hipMemGetInfo). */
+int gi_mem_info(size_t *free_bytes, size_t *total_bytes);
+
+/* ---- the four step-loop kernels ----------------------------------------- */
+
+/* getForcesCollisionSingleGPU (impactAddTime.cu:245-377): for every particle i,
+ * force[i].xyz = sum over all j of force_mag(r, materials, shell, receding) * dp / r
+ * with dp = pos[j] - pos[i]; force[i].w is not written. */
+int gi_force_collision(const gi_float4 *pos, const gi_float4 *vel, gi_float4 *force, int n,
+                       const gi_force_collision_constants *c, gi_stream_t stream);
+
+/* moveBodiesCollisionSingleGPU (impactAddTime.cu:379-394):
+ * vel += force * (Dt/m); pos += vel * Dt  (in that order; .w untouched). */
+int gi_move_collision(gi_float4 *pos, gi_float4 *vel, const gi_float4 *force, int n,
+                      const gi_move_collision_constants *c, gi_stream_t stream);
+
+/* getForcesSeperate (generateTargetImpactor.cu:486-626): the same pair force,
+ * but only between particles of the same body ([0,boarder2) and [boarder2,n)). */
+int gi_force_separate(const gi_float4 *pos, const gi_float4 *vel, gi_float4 *force, int n,
+                      const gi_force_separate_constants *c, gi_stream_t stream);
+
+/* moveBodiesDampedSeperate (generateTargetImpactor.cu:628-664):
+ * vel += (force - damp*vel) * (Dt/m); pos += vel * Dt, with damp = DampRateBody1
+ * for body 1 and DampRateBody2 for body 2. */
+int gi_move_damped_separate(gi_float4 *pos, gi_float4 *vel, const gi_float4 *force, int n,
+                            const gi_move_separate_constants *c, float damp_rate_body1,
+                            float damp_rate_body2, gi_stream_t stream);
+
+/* ---- whole-system reductions (FP64) -------------------------------------- */
+
+/* Mass-weighted centre of mass and mean velocity of particles [0,n) with
+ * iron for id < nfe, in FP64 (the reference's getCenterOfMassCollision(0) /
+ * getLinearVelocityCollision(0), impactBasicFunctions.h:268-297,388-417, sum
+ * in float on the host).  out6 = {cmx, cmy, cmz, vx, vy, vz}; blocking. */
+int gi_center_of_mass_velocity(const gi_float4 *pos, const gi_float4 *vel, int n, int nfe,
+                               double mass_fe, double mass_si, double *out6, gi_stream_t stream);
+
+/* Recenter (impactAddTime.cu:433-455): shift every position by
+ * (target_com - com) and every velocity by (target_vel - mean_vel), where com
+ * and mean_vel are the current mass-weighted means; entirely on the device. */
+int gi_recenter(gi_float4 *pos, gi_float4 *vel, int n, int nfe, double mass_fe, double mass_si,
+                const double *target_com3, const double *target_vel3, gi_stream_t stream);
+
+/* FP64 conserved quantities of one simulation (particles [0,n), iron id < nfe):
+ * out[0] kinetic, out[1] potential, out[2] total energy,
+ * out[3..5] linear momentum, out[6..8] angular momentum about the centre of mass.
+ * The potential of a pair is -G mi mj / r for r >= 1 and
+ * G mi mj (r - 2) + (K/3)(r - 1)^2 (r + 2) for r < 1 (K = approach stiffness of
+ * the pair), whose derivative is the force law.  Blocking. */
+int gi_conserved_quantities(const gi_float4 *pos, const gi_float4 *vel, int n,
+                            const gi_force_collision_constants *c, double *out9,
+                            gi_stream_t stream);
+
+/* ---- native single-simulation stepper ------------------------------------ */
+
+/* nBodyCollisionSingleGPU (impactAddTime.cu:396-481) without the GL, file and
+ * host round trips: n_steps of (force, move) on device-resident state, with an
+ * optional device-side recenter every recenter_rate steps (0 = never) toward
+ * target_com3 / target_vel3.  The step sequence is captured once in a HIP
+ * graph and replayed. */
+int gi_impact_steps(gi_float4 *pos, gi_float4 *vel, gi_float4 *force, int n,
+                    const gi_force_collision_constants *fc, const gi_move_collision_constants *mc,
+                    int n_steps, int recenter_rate, const double *target_com3,
+                    const double *target_vel3, gi_stream_t stream);
+
+/* ---- batched ensemble (many impacts of uneven n in one launch per kernel) - */
+
+typedef struct gi_ensemble gi_ensemble;
+
+/* Create an ensemble of n_sims independent impacts.  Particle arrays are
+ * packed back to back: simulation s owns particles [offset_s, offset_s + n[s])
+ * with offset_s = n[0] + ... + n[s-1], each in the impact layout (iron first,
+ * fc[s].NFe irons).  pos/vel are device pointers owned by the caller; force is
+ * owned by the ensemble.  tile_config may be NULL (autotuned defaults). */
+int gi_ensemble_create(gi_ensemble **out, int n_sims, const int *n,
+                       const gi_force_collision_constants *fc,
+                       const gi_move_collision_constants *mc, gi_float4 *pos, gi_float4 *vel,
+                       gi_stream_t stream);
+int gi_ensemble_destroy(gi_ensemble *e);
+/* Advance every simulation whose active flag is set by n_steps steps. */
+int gi_ensemble_steps(gi_ensemble *e, int n_steps);
+/* Per-simulation active flags (host array of n_sims ints; 0 = frozen). */
+int gi_ensemble_set_active(gi_ensemble *e, const int *active);
+/* FP64 conserved quantities per simulation: out is n_sims x 9 doubles (see
+ * gi_conserved_quantities).  Blocking. */
+int gi_ensemble_conserved(gi_ensemble *e, double *out);
+/* Total particles and total pair interactions of one step of the active set. */
+int gi_ensemble_work(gi_ensemble *e, long long *particles, double *pairs);
+
+/* ---- kernel timing (HIP events around the all-pairs force kernel) ------- */
+
+/* Enable/disable per-launch timing of the all-pairs gravity kernel; when on,
+ * gi_kernel_timing returns the launch count and the summed milliseconds since
+ * the last reset. */
+int gi_set_kernel_timing(int enable);
+int gi_kernel_timing(const char *kernel, long long *launches, double *total_ms);
+int gi_reset_kernel_timing(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GIANT_IMPACT_H */
