@@ -1,0 +1,200 @@
+#!/usr/bin/env python3
+"""Benchmark of the Giant-Impact step loop on one MI355X.
+
+Workload: one impact of N = 262,144 elements (196,608-element target and a
+65,536-element impactor, 30% iron cores, jittered cubic lattices, 45 degree
+impact at 1.1 v_escape), FP32.  A step is one force evaluation
+(gi_force_collision: all-pairs gravity + contact grid + exact contact
+correction) and one move (gi_move_collision) over every particle, with the
+state resident in HBM.
+
+Printed: one JSON line.  value = pair interactions per second of whole steps
+(N^2 per step, self pairs included, as the reference kernel visits them).
+roofline: the all-pairs gravity kernel, 20 FP32 flops per pair against the
+157.3 TFLOP/s FP32 peak, timed with HIP events on its stream during the
+timed steps.  cpu_baseline: the oracle's float restatement of the reference
+kernel timed on this host's cores over a bounded sample of rows.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+REPO = Path(__file__).resolve().parent
+sys.path.insert(0, str(REPO))
+sys.path.insert(0, str(REPO / "giant-impact_amd"))
+
+FLOPS_PER_PAIR = 20
+PEAK_FP32_TFLOPS = 157.3
+PEAK_HBM_GBPS = 8000.0
+MOVE_BYTES_PER_PARTICLE = 80  # pos, vel, force in; pos, vel out
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def traffic_from_profiles(kernel_prefix: str):
+    """HBM bytes per launch measured with rocprofv3 counters (profiles/*.txt line
+    'hbm_bytes_per_launch <kernel> <n> <bytes>'), or None."""
+    for f in sorted((REPO / "profiles").glob("*.txt")):
+        for line in f.read_text().splitlines():
+            parts = line.split()
+            if len(parts) == 4 and parts[0] == "hbm_bytes_per_launch" and parts[1] == kernel_prefix:
+                return int(parts[2]), float(parts[3])
+    return None
+
+
+def cpu_baseline(pos, vel, fc, n, target_s: float):
+    from oracle import oracle as O
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    rows = 64
+    t0 = time.perf_counter()
+    O.force_collision_f32(pos, vel, fc, 0, rows, nthreads=threads)
+    dt = time.perf_counter() - t0
+    rows = int(min(n, max(rows, rows * target_s / max(dt, 1e-6))))
+    t0 = time.perf_counter()
+    O.force_collision_f32(pos, vel, fc, 0, rows, nthreads=threads)
+    dt = time.perf_counter() - t0
+    return {
+        "value": rows * n / dt,
+        "unit": "pair-int/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"rows [0,{rows}) x all {n} j of one force evaluation of the N={n} impact "
+        f"({rows * n:.3e} pairs, {dt:.1f} s) with the float restatement of getForcesCollisionSingleGPU, "
+        f"gcc -O2 -ffp-contract=off, OpenMP over rows",
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n-target", type=int, default=196608)
+    ap.add_argument("--n-impactor", type=int, default=65536)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(
+            "bench.py measures one GPU: this build runs on a single MI355X and has no multi-GPU path "
+            "(the sharding design is described in DESIGN.md). Use --gpus 1.",
+            file=sys.stderr,
+        )
+        return 2
+
+    import gi_amd as gi
+
+    if gi.device_count() < 1:
+        print("bench.py: no HIP device visible", file=sys.stderr)
+        return 1
+    lib = gi.load()
+
+    nt, ni = args.n_target, args.n_impactor
+    n = nt + ni
+    pos, vel, fc, mc, info = gi.make_impact(nt, ni, seed=20261202 + 2, angle_deg=45.0, speed_vesc=1.1)
+    log(f"N={n} NFe={fc.NFe} R_t={info.radius_target:.2f} v_esc={info.v_escape:.3f}")
+    P, V, F = gi.DeviceArray(n, pos), gi.DeviceArray(n, vel), gi.DeviceArray(n, np.zeros((n, 4), np.float32))
+
+    # parity spot check on the initial state (rows spread over all materials)
+    from oracle import oracle as O
+
+    gi.force_collision(P, V, F, n, fc)
+    gi.sync()
+    Fg = F.download()
+    rows = []
+    for a in (0, fc.NFe // 2, fc.NFe, n // 2, n - 64):
+        F64, cond, near = O.force_collision_f64(pos, vel, fc, i_begin=a, i_end=a + 64)
+        e = np.linalg.norm(Fg[a : a + 64, :3].astype(np.float64) - F64[a : a + 64], axis=1)
+        ok = ~near[a : a + 64]
+        rows.append((np.max(e[ok] / cond[a : a + 64][ok]), np.max(e[ok] / np.linalg.norm(F64[a : a + 64][ok], axis=1))))
+    parity = {"rows_checked": 320, "cond_err_max": float(max(r[0] for r in rows)), "rel_err_max": float(max(r[1] for r in rows))}
+    log(f"parity vs FP64 oracle: {parity}")
+
+    for _ in range(args.warmup):
+        gi.force_collision(P, V, F, n, fc)
+        gi.move_collision(P, V, F, n, mc)
+    gi.sync()
+
+    lib.gi_set_kernel_timing(1)
+    lib.gi_reset_kernel_timing()
+    gi.sync()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        gi.force_collision(P, V, F, n, fc)
+        gi.move_collision(P, V, F, n, mc)
+    gi.sync()
+    t1 = time.perf_counter()
+    lib.gi_set_kernel_timing(0)
+    elapsed = t1 - t0
+    L, T = C.c_longlong(), C.c_double()
+    gi.call("gi_kernel_timing", b"gravity", C.byref(L), C.byref(T))
+    grav_s = T.value / 1e3 / max(L.value, 1)
+    gi.call("gi_kernel_timing", b"move", C.byref(L), C.byref(T))
+    move_s = T.value / 1e3 / max(L.value, 1)
+    log(f"{args.steps} steps in {elapsed:.3f} s; gravity kernel {grav_s * 1e3:.3f} ms; move kernel {move_s * 1e6:.1f} us")
+
+    pairs = float(n) * float(n)
+    value = pairs * args.steps / elapsed
+    achieved = pairs * FLOPS_PER_PAIR / grav_s / 1e12
+    tr = traffic_from_profiles("k_gravity")
+    traffic = int(tr[1]) if tr and tr[0] == n else None
+    result = {
+        "metric": "pair-int/s",
+        "value": value,
+        "unit": "pair-int/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded jittered-lattice impact)",
+        "config": {
+            "workload": f"single impact N={n} ({nt} target + {ni} impactor, 30% Fe, 45 deg, 1.1 v_esc), "
+            "one force + move step",
+            "n": n,
+            "n_fe": int(fc.NFe),
+            "flops_per_pair": FLOPS_PER_PAIR,
+        },
+        "roofline": {
+            "bound": "valu",
+            "kernel": "k_gravity",
+            "achieved": achieved,
+            "peak": PEAK_FP32_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / PEAK_FP32_TFLOPS,
+            "traffic": traffic,
+            "pairs_per_s": pairs / grav_s,
+            "avg_launch_ms": grav_s * 1e3,
+        },
+        "move_kernel": {
+            "avg_launch_us": move_s * 1e6,
+            "achieved_GBps": MOVE_BYTES_PER_PARTICLE * n / move_s / 1e9,
+            "peak_GBps": PEAK_HBM_GBPS,
+            "bytes_per_launch": MOVE_BYTES_PER_PARTICLE * n,
+        },
+        "parity": parity,
+    }
+    if not args.no_cpu:
+        log("timing the CPU baseline")
+        result["cpu_baseline"] = cpu_baseline(pos, vel, fc, n, args.cpu_seconds)
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

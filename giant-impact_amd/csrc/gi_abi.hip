@@ -45,21 +45,25 @@ static int set_error(int code, const char* fmt, ...) {
     } while (0)
 
 // ---------------------------------------------------------------------------
-// Per-device scratch
+// Device scratch
 // ---------------------------------------------------------------------------
 
 template <typename T>
 struct DevBuf {
     T* p = nullptr;
     size_t cap = 0;  // elements
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
     int ensure(size_t n) {
         if (n <= cap && p) return 0;
-        if (p) {
-            hipError_t e = hipFree(p);
-            p = nullptr;
-            cap = 0;
-            if (e != hipSuccess) return set_error((int)e, "hipFree: %s", hipGetErrorString(e));
-        }
+        release();
         size_t want = n < 64 ? 64 : n;
         hipError_t e = hipMalloc((void**)&p, want * sizeof(T));
         if (e != hipSuccess) {
@@ -71,8 +75,11 @@ struct DevBuf {
     }
 };
 
+// HIP events around one timed launch; kind 0 = all-pairs gravity, 1 = move.
+constexpr int kTimedKinds = 2;
 struct TimingPair {
     hipEvent_t a = nullptr, b = nullptr;
+    int kind = 0;
 };
 
 // The plan of one force evaluation over a set of packed simulations.
@@ -80,28 +87,50 @@ struct Plan {
     std::vector<SimDesc> sims;
     std::vector<GravTask> tasks;
     std::vector<int> block_sim, block_first, sim_first_block;
+    Tiling tiling{4, 0, 0, 0};
     int ntot = 0;
-    int R = 4;
     int partial_len = 0;  // float4 elements
     unsigned mask = 0;    // hash table size - 1
 };
 
-struct DevCtx {
-    int device = -1;
+// Scratch and plan of one owner (the drop-in entry points of a device, or
+// one ensemble).
+struct Workspace {
     DevBuf<float4> partial;
     DevBuf<int> counts, starts, sorted, cellhash, block_sums;
     DevBuf<GravTask> tasks;
     DevBuf<SimDesc> sims;
     DevBuf<int> block_sim, block_first, sim_first_block;
-    DevBuf<double> moments, sums, pe, target;
-    // cached plan (uploaded copy)
+    DevBuf<double> moments, sums, pe;
     Plan plan;
     bool plan_valid = false;
-    // timing
-    std::vector<TimingPair> pending;
-    std::vector<TimingPair> pool;
-    long long timed_launches = 0;
-    double timed_ms = 0.0;
+    unsigned long long generation = 0;  // bumped on every re-plan
+};
+
+// A captured step sequence and what it was captured for.
+struct GraphEntry {
+    std::vector<unsigned char> key;
+    unsigned long long generation = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    void destroy() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+    }
+};
+
+struct DevCtx {
+    int device = -1;
+    Workspace ws;
+    DevBuf<double> rc_blocks, rc_sums;
+    hipStream_t capture_stream = nullptr;
+    hipEvent_t ev_in = nullptr, ev_out = nullptr;
+    std::vector<GraphEntry> graphs;
+    std::vector<TimingPair> pending, pool;
+    long long timed_launches[kTimedKinds] = {0, 0};
+    double timed_ms[kTimedKinds] = {0.0, 0.0};
 };
 
 static std::mutex g_ctx_mutex;
@@ -128,39 +157,32 @@ static int current_ctx(DevCtx** out) {
 
 static PairLaw law_of(const gi_force_collision_constants* c) {
     PairLaw L;
-    L.GMassFeFe = c->GMassFeFe;
-    L.GMassFeSi = c->GMassFeSi;
-    L.KFeFe = c->KFeFe;
-    L.KSiSi = c->KSiSi;
-    L.KFeSi = c->KFeSi;
-    L.KRFeFe = c->KRFeFe;
-    L.KRSiSi = c->KRSiSi;
-    L.KRFeSi = c->KRFeSi;
-    L.KRMix = c->KRMix;
-    L.ShellBreakFe = c->ShellBreakFe;
-    L.ShellBreakSi = c->ShellBreakSi;
-    L.ShellBreakFeSi1 = c->ShellBreakFeSi1;
-    L.ShellBreakFeSi2 = c->ShellBreakFeSi2;
+    std::memcpy(&L, c, sizeof(PairLaw));  // the 13 leading floats, in struct order
     return L;
 }
 
 static PairLaw law_of(const gi_force_separate_constants* c) {
-    gi_force_collision_constants t;
-    std::memcpy(&t, c, 13 * sizeof(float));
-    t.NFe = 0;
-    return law_of(&t);
+    PairLaw L;
+    std::memcpy(&L, c, sizeof(PairLaw));
+    return L;
 }
+static_assert(sizeof(PairLaw) == 13 * sizeof(float), "PairLaw mirrors the 13 leading floats");
 
 static bool same_sims(const std::vector<SimDesc>& a, const std::vector<SimDesc>& b) {
     return a.size() == b.size() && std::memcmp(a.data(), b.data(), a.size() * sizeof(SimDesc)) == 0;
 }
 
+static bool same_tiling(const Tiling& a, const Tiling& b) {
+    return a.R == b.R && a.chunk_len == b.chunk_len && a.target_tasks == b.target_tasks &&
+           a.min_chunk == b.min_chunk;
+}
+
 // Fill nchunk / partial_base of the simulations and build the task table.
 static int build_plan(std::vector<SimDesc> sims, const Tiling& tl, Plan* out) {
     Plan p;
-    p.R = tl.R;
+    p.tiling = tl;
     long long ntot = 0;
-    long long work = 0;  // sum over sims of iblocks * n (pair-slots)
+    long long work = 0;  // sum over active sims of iblocks * n
     const int ib = 64 * tl.R;
     for (auto& s : sims) {
         long long n = s.end - s.start;
@@ -171,8 +193,9 @@ static int build_plan(std::vector<SimDesc> sims, const Tiling& tl, Plan* out) {
     }
     // The contact-grid scan handles up to 4096 * 1024 buckets = 2 * 2^21.
     if (ntot > (1LL << 21)) return set_error(GI_E_TOO_LARGE, "too many particles (%lld > 2^21)", ntot);
-    long long L = tl.chunk_len > 0 ? tl.chunk_len : work / (long long)tl.target_tasks;
+    long long L = tl.chunk_len > 0 ? tl.chunk_len : work / (tl.target_tasks > 0 ? tl.target_tasks : 1);
     if (L < tl.min_chunk) L = tl.min_chunk;
+    if (L < 64) L = 64;
     L = (L + 63) / 64 * 64;
     long long pbase = 0;
     for (size_t si = 0; si < sims.size(); ++si) {
@@ -211,8 +234,9 @@ static int build_plan(std::vector<SimDesc> sims, const Tiling& tl, Plan* out) {
         pbase += (long long)chunk * n;
         if (pbase > (1LL << 31) - 1) return set_error(GI_E_TOO_LARGE, "partial buffer too large");
     }
-    // Interleave tasks so consecutive wavefronts share a j-chunk (L2 reuse on
-    // each XCD) while different i-blocks spread over the 8 XCDs.
+    // Longest tasks first, so the tail of the launch is made of short ones.
+    std::stable_sort(p.tasks.begin(), p.tasks.end(),
+                     [](const GravTask& a, const GravTask& b) { return (a.j1 - a.j0) > (b.j1 - b.j0); });
     p.partial_len = (int)pbase;
     p.ntot = (int)ntot;
     unsigned m = 1024;
@@ -240,33 +264,50 @@ static int upload(DevBuf<T>& b, const std::vector<T>& v, hipStream_t st) {
     return 0;
 }
 
-// Make ctx->plan describe `sims` (re-planning and uploading only on change).
-static int ensure_plan(DevCtx* ctx, const std::vector<SimDesc>& sims, hipStream_t st) {
+// Make ws->plan describe `sims` (re-planning and uploading only on change).
+static int ensure_plan(Workspace* ws, const std::vector<SimDesc>& sims, hipStream_t st) {
     long long ntot = sims.empty() ? 0 : sims.back().end;
     long long pairs = 0;
-    for (auto& s : sims) pairs += (long long)(s.end - s.start) * (s.end - s.start);
+    for (auto& s : sims)
+        if (s.active) pairs += (long long)(s.end - s.start) * (s.end - s.start);
     Tiling tl = choose_tiling(ntot, pairs, (int)sims.size());
-    if (ctx->plan_valid && same_sims(ctx->plan.sims, sims) && ctx->plan.R == tl.R) return 0;
+    if (ws->plan_valid && same_sims(ws->plan.sims, sims) && same_tiling(ws->plan.tiling, tl)) {
+        // same simulations; only partial_base/nchunk were filled in by the planner
+        return 0;
+    }
+    // same_sims compares against the planned copy, whose nchunk/partial_base
+    // are filled in: compare on the caller's fields only.
+    if (ws->plan_valid && same_tiling(ws->plan.tiling, tl) && ws->plan.sims.size() == sims.size()) {
+        bool same = true;
+        for (size_t k = 0; k < sims.size() && same; ++k) {
+            SimDesc a = sims[k], b = ws->plan.sims[k];
+            a.nchunk = b.nchunk;
+            a.partial_base = b.partial_base;
+            same = std::memcmp(&a, &b, sizeof a) == 0;
+        }
+        if (same) return 0;
+    }
     Plan p;
     int rc = build_plan(sims, tl, &p);
     if (rc) return rc;
-    if ((rc = ctx->partial.ensure((size_t)p.partial_len + 1))) return rc;
-    if ((rc = ctx->counts.ensure((size_t)p.mask + 1))) return rc;
-    if ((rc = ctx->starts.ensure((size_t)p.mask + 2))) return rc;
-    if ((rc = ctx->sorted.ensure((size_t)p.ntot + 1))) return rc;
-    if ((rc = ctx->cellhash.ensure((size_t)p.ntot + 1))) return rc;
-    if ((rc = ctx->block_sums.ensure(((size_t)p.mask + 1) / 1024 + 8))) return rc;
-    if ((rc = upload(ctx->tasks, p.tasks, st))) return rc;
-    if ((rc = upload(ctx->sims, p.sims, st))) return rc;
-    if ((rc = upload(ctx->block_sim, p.block_sim, st))) return rc;
-    if ((rc = upload(ctx->block_first, p.block_first, st))) return rc;
-    if ((rc = upload(ctx->sim_first_block, p.sim_first_block, st))) return rc;
-    if ((rc = ctx->moments.ensure(p.block_sim.size() * kMoments + 1))) return rc;
-    if ((rc = ctx->sums.ensure(p.sims.size() * kMoments + 1))) return rc;
-    if ((rc = ctx->pe.ensure((size_t)p.ntot + 1))) return rc;
-    if ((rc = ctx->target.ensure(8))) return rc;
-    ctx->plan = std::move(p);
-    ctx->plan_valid = true;
+    if ((rc = ws->partial.ensure((size_t)p.partial_len + 1))) return rc;
+    if ((rc = ws->counts.ensure((size_t)p.mask + 1))) return rc;
+    if ((rc = ws->starts.ensure((size_t)p.mask + 2))) return rc;
+    if ((rc = ws->sorted.ensure((size_t)p.ntot + 1))) return rc;
+    if ((rc = ws->cellhash.ensure((size_t)p.ntot + 1))) return rc;
+    if ((rc = ws->block_sums.ensure(((size_t)p.mask + 1) / 1024 + 8))) return rc;
+    if ((rc = upload(ws->tasks, p.tasks, st))) return rc;
+    if ((rc = upload(ws->sims, p.sims, st))) return rc;
+    if ((rc = upload(ws->block_sim, p.block_sim, st))) return rc;
+    if ((rc = upload(ws->block_first, p.block_first, st))) return rc;
+    if ((rc = upload(ws->sim_first_block, p.sim_first_block, st))) return rc;
+    if ((rc = ws->moments.ensure(p.block_sim.size() * kMoments + 1))) return rc;
+    if ((rc = ws->sums.ensure(p.sims.size() * kMoments + 1))) return rc;
+    if ((rc = ws->pe.ensure((size_t)p.ntot + 1))) return rc;
+    // The uploads read host vectors that live on in ws->plan.
+    ws->plan = std::move(p);
+    ws->plan_valid = true;
+    ws->generation++;
     return 0;
 }
 
@@ -276,83 +317,137 @@ static inline int nblk(long long n, int b) { return (int)((n + b - 1) / b); }
 // Launch sequences
 // ---------------------------------------------------------------------------
 
-static int launch_grid(DevCtx* ctx, const float4* pos, hipStream_t st) {
-    const Plan& p = ctx->plan;
+static int launch_grid(Workspace* ws, const float4* pos, hipStream_t st) {
+    const Plan& p = ws->plan;
     const int m = (int)p.mask + 1;
     const int nsims = (int)p.sims.size();
-    hipLaunchKernelGGL(k_fill_int, dim3(nblk(m, 256)), dim3(256), 0, st, ctx->counts.p, m, 0);
-    hipLaunchKernelGGL(k_grid_count, dim3(nblk(p.ntot, 256)), dim3(256), 0, st, pos, p.ntot,
-                       ctx->sims.p, nsims, p.mask, ctx->counts.p, ctx->cellhash.p);
+    hipLaunchKernelGGL(k_fill_int, dim3(nblk(m, 256)), dim3(256), 0, st, ws->counts.p, m, 0);
+    hipLaunchKernelGGL(k_grid_count, dim3(nblk(p.ntot, 256)), dim3(256), 0, st, pos, p.ntot, ws->sims.p,
+                       nsims, p.mask, ws->counts.p, ws->cellhash.p);
     const int nb = nblk(m, 1024);
-    hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(256), 0, st, ctx->counts.p, m, ctx->starts.p,
-                       ctx->block_sums.p);
-    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, st, ctx->block_sums.p, nb, ctx->starts.p + m);
-    hipLaunchKernelGGL(k_scan_add, dim3(nblk(m, 256)), dim3(256), 0, st, ctx->starts.p, m,
-                       ctx->block_sums.p);
-    hipLaunchKernelGGL(k_grid_scatter, dim3(nblk(p.ntot, 256)), dim3(256), 0, st, p.ntot,
-                       ctx->cellhash.p, ctx->starts.p, ctx->counts.p, ctx->sorted.p);
-    hipLaunchKernelGGL(k_grid_sort_buckets, dim3(nblk(m, 256)), dim3(256), 0, st, m, ctx->starts.p,
-                       ctx->sorted.p);
+    hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(256), 0, st, ws->counts.p, m, ws->starts.p,
+                       ws->block_sums.p);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, st, ws->block_sums.p, nb, ws->starts.p + m);
+    hipLaunchKernelGGL(k_scan_add, dim3(nblk(m, 256)), dim3(256), 0, st, ws->starts.p, m, ws->block_sums.p);
+    hipLaunchKernelGGL(k_grid_scatter, dim3(nblk(p.ntot, 256)), dim3(256), 0, st, p.ntot, ws->cellhash.p,
+                       ws->starts.p, ws->counts.p, ws->sorted.p);
+    hipLaunchKernelGGL(k_grid_sort_buckets, dim3(nblk(m, 256)), dim3(256), 0, st, m, ws->starts.p,
+                       ws->sorted.p);
     GI_HIP(hipGetLastError());
     return 0;
 }
 
-static TimingPair take_pair(DevCtx* ctx) {
+static TimingPair take_pair(DevCtx* ctx, int kind) {
     TimingPair t;
     if (!ctx->pool.empty()) {
         t = ctx->pool.back();
         ctx->pool.pop_back();
-        return t;
+    } else {
+        (void)hipEventCreate(&t.a);
+        (void)hipEventCreate(&t.b);
     }
-    (void)hipEventCreate(&t.a);
-    (void)hipEventCreate(&t.b);
+    t.kind = kind;
     return t;
 }
 
-static int launch_gravity(DevCtx* ctx, const float4* pos, bool pot, hipStream_t st, bool timed) {
-    const Plan& p = ctx->plan;
+template <bool POT>
+static void launch_gravity_kernel(int R, dim3 grid, dim3 block, hipStream_t st, const float4* pos,
+                                  const Workspace* ws, int ntasks) {
+    if (R == 8)
+        hipLaunchKernelGGL((k_gravity<8, 128, POT>), grid, block, 0, st, pos, ws->tasks.p, ntasks, ws->sims.p,
+                           ws->partial.p);
+    else if (R == 2)
+        hipLaunchKernelGGL((k_gravity<2, 128, POT>), grid, block, 0, st, pos, ws->tasks.p, ntasks, ws->sims.p,
+                           ws->partial.p);
+    else
+        hipLaunchKernelGGL((k_gravity<4, 128, POT>), grid, block, 0, st, pos, ws->tasks.p, ntasks, ws->sims.p,
+                           ws->partial.p);
+}
+
+static int launch_gravity(DevCtx* ctx, Workspace* ws, const float4* pos, bool pot, hipStream_t st,
+                          bool timed) {
+    const Plan& p = ws->plan;
     const int ntasks = (int)p.tasks.size();
     if (ntasks == 0) return 0;
     const int wpb = 4;
     dim3 grid(nblk(ntasks, wpb)), block(64 * wpb);
     TimingPair tp;
     if (timed) {
-        tp = take_pair(ctx);
-        (void)hipEventRecord(tp.a, st);
+        tp = take_pair(ctx, 0);
+        GI_HIP(hipEventRecord(tp.a, st));
     }
-    if (pot) {
-        hipLaunchKernelGGL((k_gravity<4, 128, true>), grid, block, 0, st, pos, ctx->tasks.p, ntasks,
-                           ctx->sims.p, ctx->partial.p);
-    } else if (p.R == 8) {
-        hipLaunchKernelGGL((k_gravity<8, 128, false>), grid, block, 0, st, pos, ctx->tasks.p, ntasks,
-                           ctx->sims.p, ctx->partial.p);
-    } else if (p.R == 2) {
-        hipLaunchKernelGGL((k_gravity<2, 128, false>), grid, block, 0, st, pos, ctx->tasks.p, ntasks,
-                           ctx->sims.p, ctx->partial.p);
-    } else {
-        hipLaunchKernelGGL((k_gravity<4, 128, false>), grid, block, 0, st, pos, ctx->tasks.p, ntasks,
-                           ctx->sims.p, ctx->partial.p);
-    }
+    if (pot)
+        launch_gravity_kernel<true>(p.tiling.R, grid, block, st, pos, ws, ntasks);
+    else
+        launch_gravity_kernel<false>(p.tiling.R, grid, block, st, pos, ws, ntasks);
     GI_HIP(hipGetLastError());
     if (timed) {
-        (void)hipEventRecord(tp.b, st);
+        GI_HIP(hipEventRecord(tp.b, st));
         ctx->pending.push_back(tp);
     }
     return 0;
 }
 
 // Full force evaluation for the current plan: grid, all pairs, finish.
-static int run_force(DevCtx* ctx, const float4* pos, const float4* vel, float4* force,
+static int run_force(DevCtx* ctx, Workspace* ws, const float4* pos, const float4* vel, float4* force,
                      hipStream_t st, bool timed) {
-    const Plan& p = ctx->plan;
+    const Plan& p = ws->plan;
     if (p.ntot == 0) return 0;
-    int rc = launch_grid(ctx, pos, st);
+    int rc = launch_grid(ws, pos, st);
     if (rc) return rc;
-    // A plan with the R of the pot variant must match: the pot variant uses R=4 tasks.
-    if ((rc = launch_gravity(ctx, pos, false, st, timed))) return rc;
-    hipLaunchKernelGGL(k_finish, dim3(nblk(p.ntot, 256)), dim3(256), 0, st, pos, vel, force, p.ntot,
-                       ctx->sims.p, (int)p.sims.size(), ctx->partial.p, ctx->starts.p,
-                       ctx->sorted.p, p.mask);
+    if ((rc = launch_gravity(ctx, ws, pos, false, st, timed))) return rc;
+    hipLaunchKernelGGL(k_finish, dim3(nblk(p.ntot, 256)), dim3(256), 0, st, pos, vel, force, p.ntot, ws->sims.p,
+                       (int)p.sims.size(), ws->partial.p, ws->starts.p, ws->sorted.p, p.mask);
+    GI_HIP(hipGetLastError());
+    return 0;
+}
+
+// Per-simulation FP64 moments (with the potential) into ws->sums.
+static int run_conserved(DevCtx* ctx, Workspace* ws, const float4* pos, const float4* vel, hipStream_t st) {
+    const Plan& p = ws->plan;
+    if (p.ntot == 0) return 0;
+    int rc = launch_grid(ws, pos, st);
+    if (rc) return rc;
+    if ((rc = launch_gravity(ctx, ws, pos, true, st, false))) return rc;
+    hipLaunchKernelGGL(k_finish_potential, dim3(nblk(p.ntot, 256)), dim3(256), 0, st, pos, p.ntot, ws->sims.p,
+                       (int)p.sims.size(), ws->partial.p, ws->starts.p, ws->sorted.p, p.mask, ws->pe.p);
+    const int nb = (int)p.block_sim.size();
+    hipLaunchKernelGGL(k_moments, dim3(nb), dim3(256), 0, st, pos, vel, (const double*)ws->pe.p,
+                       ws->block_sim.p, ws->block_first.p, ws->sims.p, 0.0, 0.0, ws->moments.p);
+    const int ns = (int)p.sims.size();
+    hipLaunchKernelGGL(k_moments_final, dim3(nblk(ns, 64)), dim3(64), 0, st, (const double*)ws->moments.p,
+                       ws->sim_first_block.p, ns, ws->sums.p);
+    GI_HIP(hipGetLastError());
+    return 0;
+}
+
+// kinetic, potential, total, P(3), L about the centre of mass (3) from the 14 moments.
+static void moments_to_conserved(const double* m, double* out9) {
+    const double M = m[0];
+    out9[0] = m[10];
+    out9[1] = m[11];
+    out9[2] = m[10] + m[11];
+    out9[3] = m[4];
+    out9[4] = m[5];
+    out9[5] = m[6];
+    // L_com = sum m x cross v - (sum m x) cross (sum m v) / M
+    const double X[3] = {m[1], m[2], m[3]}, P[3] = {m[4], m[5], m[6]};
+    out9[6] = m[7] - (X[1] * P[2] - X[2] * P[1]) / M;
+    out9[7] = m[8] - (X[2] * P[0] - X[0] * P[2]) / M;
+    out9[8] = m[9] - (X[0] * P[1] - X[1] * P[0]) / M;
+}
+
+static int launch_recenter(DevCtx* ctx, float4* pos, float4* vel, int n, int nfe, double mfe, double msi,
+                           const double* tc, const double* tv, hipStream_t st) {
+    const int nb = nblk(n, 256);
+    int rc;
+    if ((rc = ctx->rc_blocks.ensure((size_t)nb * kMoments1 + 1))) return rc;
+    if ((rc = ctx->rc_sums.ensure(kMoments1 + 1))) return rc;
+    hipLaunchKernelGGL(k_moments_one, dim3(nb), dim3(256), 0, st, pos, vel, n, nfe, mfe, msi, ctx->rc_blocks.p);
+    hipLaunchKernelGGL(k_moments_one_final, dim3(1), dim3(64), 0, st, (const double*)ctx->rc_blocks.p, nb,
+                       ctx->rc_sums.p);
+    hipLaunchKernelGGL(k_recenter_apply, dim3(nb), dim3(256), 0, st, pos, vel, n, (const double*)ctx->rc_sums.p,
+                       tc[0], tc[1], tc[2], tv[0], tv[1], tv[2]);
     GI_HIP(hipGetLastError());
     return 0;
 }
@@ -370,7 +465,100 @@ static SimDesc make_sim(int start, int fe_end, int end, const PairLaw& L) {
     return s;
 }
 
+// ---------------------------------------------------------------------------
+// HIP graphs of the step loop
+// ---------------------------------------------------------------------------
+
+template <typename T>
+static void key_append(std::vector<unsigned char>& k, const T& v) {
+    const unsigned char* b = reinterpret_cast<const unsigned char*>(&v);
+    k.insert(k.end(), b, b + sizeof(T));
+}
+
+// Capture `body` on a private stream into a graph cached under `key`.
+template <typename F>
+static int get_graph(DevCtx* ctx, std::vector<GraphEntry>& cache, const std::vector<unsigned char>& key,
+                     unsigned long long generation, F body, hipGraphExec_t* out) {
+    for (auto& g : cache)
+        if (g.key == key && g.generation == generation) {
+            *out = g.exec;
+            return 0;
+        }
+    if (!ctx->capture_stream) GI_HIP(hipStreamCreateWithFlags(&ctx->capture_stream, hipStreamNonBlocking));
+    GraphEntry e;
+    e.key = key;
+    e.generation = generation;
+    GI_HIP(hipStreamBeginCapture(ctx->capture_stream, hipStreamCaptureModeThreadLocal));
+    int rc = body(ctx->capture_stream);
+    hipGraph_t g = nullptr;
+    hipError_t ce = hipStreamEndCapture(ctx->capture_stream, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (ce != hipSuccess) return set_error((int)ce, "hipStreamEndCapture: %s", hipGetErrorString(ce));
+    e.graph = g;
+    hipError_t ie = hipGraphInstantiate(&e.exec, g, nullptr, nullptr, 0);
+    if (ie != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return set_error((int)ie, "hipGraphInstantiate: %s", hipGetErrorString(ie));
+    }
+    if (cache.size() >= 8) {
+        cache.front().destroy();
+        cache.erase(cache.begin());
+    }
+    cache.push_back(e);
+    *out = e.exec;
+    return 0;
+}
+
+// Run graph launches on `st`; the legacy default stream cannot be captured
+// or used with a non-blocking capture stream without explicit ordering, so
+// work for st == NULL goes to the capture stream between two events.
+struct StreamBridge {
+    DevCtx* ctx;
+    hipStream_t user, run;
+    int begin(DevCtx* c, hipStream_t u) {
+        ctx = c;
+        user = u;
+        run = u;
+        if (u) return 0;
+        if (!c->capture_stream) GI_HIP(hipStreamCreateWithFlags(&c->capture_stream, hipStreamNonBlocking));
+        if (!c->ev_in) GI_HIP(hipEventCreateWithFlags(&c->ev_in, hipEventDisableTiming));
+        if (!c->ev_out) GI_HIP(hipEventCreateWithFlags(&c->ev_out, hipEventDisableTiming));
+        GI_HIP(hipEventRecord(c->ev_in, nullptr));
+        GI_HIP(hipStreamWaitEvent(c->capture_stream, c->ev_in, 0));
+        run = c->capture_stream;
+        return 0;
+    }
+    int end() {
+        if (user) return 0;
+        GI_HIP(hipEventRecord(ctx->ev_out, run));
+        GI_HIP(hipStreamWaitEvent(nullptr, ctx->ev_out, 0));
+        return 0;
+    }
+};
+
 }  // namespace gi
+
+// ===========================================================================
+// Ensemble
+// ===========================================================================
+
+struct gi_ensemble {
+    int device = -1;
+    gi::Workspace ws;       // the stepping plan (active simulations)
+    gi::Workspace ws_diag;  // all simulations, for the conserved quantities
+    std::vector<gi::SimDesc> sims;
+    float4* pos = nullptr;
+    float4* vel = nullptr;
+    gi::DevBuf<float4> force;
+    hipStream_t stream = nullptr;
+    std::vector<gi::GraphEntry> graphs;
+    ~gi_ensemble() {
+        for (auto& g : graphs) g.destroy();
+    }
+};
 
 using namespace gi;
 
@@ -397,8 +585,7 @@ int gi_set_device(int device) {
 
 int gi_check_block_multiple(int n) {
     if (n % 256 != 0)
-        return set_error(GI_E_NOT_MULTIPLE_OF_BLOCK,
-                         "Number of Particles is not a multiple of the block size");
+        return set_error(GI_E_NOT_MULTIPLE_OF_BLOCK, "Number of Particles is not a multiple of the block size");
     return 0;
 }
 
@@ -429,6 +616,7 @@ int gi_copy_d2d(void* dst, const void* src, size_t bytes, gi_stream_t stream) {
 }
 
 int gi_stream_create(gi_stream_t* stream) {
+    if (!stream) return set_error(GI_E_INVALID_ARG, "gi_stream_create: null out pointer");
     hipStream_t s;
     GI_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
     *stream = (gi_stream_t)s;
@@ -451,72 +639,347 @@ int gi_device_sync(void) {
 }
 
 int gi_mem_info(size_t* free_bytes, size_t* total_bytes) {
+    if (!free_bytes || !total_bytes) return set_error(GI_E_INVALID_ARG, "gi_mem_info: null pointer");
     GI_HIP(hipMemGetInfo(free_bytes, total_bytes));
+    return 0;
+}
+
+int gi_set_tiling(int r, long long chunk_len, long long target_tasks, long long min_chunk) {
+    if (r == 0) {
+        tiling_override().active = 0;
+        return 0;
+    }
+    if (r != 2 && r != 4 && r != 8) return set_error(GI_E_INVALID_ARG, "gi_set_tiling: r must be 2, 4 or 8");
+    if (chunk_len < 0 || target_tasks < 0 || min_chunk < 0)
+        return set_error(GI_E_INVALID_ARG, "gi_set_tiling: negative size");
+    tiling_override().active = 1;
+    tiling_override().t = Tiling{r, chunk_len, target_tasks > 0 ? target_tasks : 8192, min_chunk};
+    return 0;
+}
+
+static int check_particles(const char* fn, const void* pos, const void* vel, const void* force, int n) {
+    if (n < 0) return set_error(GI_E_INVALID_ARG, "%s: n < 0", fn);
+    if (n > 0 && (!pos || !vel || !force)) return set_error(GI_E_INVALID_ARG, "%s: null device pointer", fn);
     return 0;
 }
 
 int gi_force_collision(const gi_float4* pos, const gi_float4* vel, gi_float4* force, int n,
                        const gi_force_collision_constants* c, gi_stream_t stream) {
-    if (n < 0 || !c || (n > 0 && (!pos || !vel || !force)))
-        return set_error(GI_E_INVALID_ARG, "gi_force_collision: invalid arguments");
+    int rc = check_particles("gi_force_collision", pos, vel, force, n);
+    if (rc) return rc;
+    if (!c) return set_error(GI_E_INVALID_ARG, "gi_force_collision: null constants");
     if (c->NFe < 0 || c->NFe > n) return set_error(GI_E_INVALID_ARG, "gi_force_collision: NFe out of [0,n]");
     if (n == 0) return 0;
     DevCtx* ctx;
-    int rc = current_ctx(&ctx);
-    if (rc) return rc;
+    if ((rc = current_ctx(&ctx))) return rc;
     hipStream_t st = as_stream(stream);
     std::vector<SimDesc> sims{make_sim(0, c->NFe, n, law_of(c))};
-    if ((rc = ensure_plan(ctx, sims, st))) return rc;
-    return run_force(ctx, (const float4*)pos, (const float4*)vel, (float4*)force, st, g_timing);
+    if ((rc = ensure_plan(&ctx->ws, sims, st))) return rc;
+    return run_force(ctx, &ctx->ws, (const float4*)pos, (const float4*)vel, (float4*)force, st, g_timing);
 }
 
 int gi_force_separate(const gi_float4* pos, const gi_float4* vel, gi_float4* force, int n,
                       const gi_force_separate_constants* c, gi_stream_t stream) {
-    if (n < 0 || !c || (n > 0 && (!pos || !vel || !force)))
-        return set_error(GI_E_INVALID_ARG, "gi_force_separate: invalid arguments");
+    int rc = check_particles("gi_force_separate", pos, vel, force, n);
+    if (rc) return rc;
+    if (!c) return set_error(GI_E_INVALID_ARG, "gi_force_separate: null constants");
     if (!(0 <= c->boarder1 && c->boarder1 <= c->boarder2 && c->boarder2 <= c->boarder3 && c->boarder3 <= n))
         return set_error(GI_E_INVALID_ARG, "gi_force_separate: borders must satisfy 0<=b1<=b2<=b3<=n");
     if (n == 0) return 0;
     DevCtx* ctx;
-    int rc = current_ctx(&ctx);
-    if (rc) return rc;
+    if ((rc = current_ctx(&ctx))) return rc;
     hipStream_t st = as_stream(stream);
-    PairLaw L = law_of(c);
+    const PairLaw L = law_of(c);
     // Body 1 = [0, b2) with iron [0, b1); body 2 = [b2, n) with iron [b2, b3):
     // two independent simulations, so cross-body pairs never meet
     // (generateTargetImpactor.cu:529-532).
     std::vector<SimDesc> sims;
     sims.push_back(make_sim(0, c->boarder1, c->boarder2, L));
     sims.push_back(make_sim(c->boarder2, c->boarder3, n, L));
-    if ((rc = ensure_plan(ctx, sims, st))) return rc;
-    return run_force(ctx, (const float4*)pos, (const float4*)vel, (float4*)force, st, g_timing);
+    if ((rc = ensure_plan(&ctx->ws, sims, st))) return rc;
+    return run_force(ctx, &ctx->ws, (const float4*)pos, (const float4*)vel, (float4*)force, st, g_timing);
 }
 
 int gi_move_collision(gi_float4* pos, gi_float4* vel, const gi_float4* force, int n,
                       const gi_move_collision_constants* c, gi_stream_t stream) {
-    if (n < 0 || !c || (n > 0 && (!pos || !vel || !force)))
-        return set_error(GI_E_INVALID_ARG, "gi_move_collision: invalid arguments");
+    int rc = check_particles("gi_move_collision", pos, vel, force, n);
+    if (rc) return rc;
+    if (!c) return set_error(GI_E_INVALID_ARG, "gi_move_collision: null constants");
     if (n == 0) return 0;
-    hipLaunchKernelGGL(k_move_collision, dim3(nblk(n, 256)), dim3(256), 0, as_stream(stream),
-                       (float4*)pos, (float4*)vel, (const float4*)force, n, c->Dt, c->DtOverMassFe,
-                       c->DtOverMassSi, c->NFe);
+    hipStream_t st = as_stream(stream);
+    DevCtx* ctx = nullptr;
+    TimingPair tp;
+    if (g_timing) {
+        if ((rc = current_ctx(&ctx))) return rc;
+        tp = take_pair(ctx, 1);
+        GI_HIP(hipEventRecord(tp.a, st));
+    }
+    hipLaunchKernelGGL(k_move_collision, dim3(nblk(n, 256)), dim3(256), 0, st, (float4*)pos, (float4*)vel,
+                       (const float4*)force, n, c->Dt, c->DtOverMassFe, c->DtOverMassSi, c->NFe);
     GI_HIP(hipGetLastError());
+    if (g_timing) {
+        GI_HIP(hipEventRecord(tp.b, st));
+        ctx->pending.push_back(tp);
+    }
     return 0;
 }
 
 int gi_move_damped_separate(gi_float4* pos, gi_float4* vel, const gi_float4* force, int n,
-                            const gi_move_separate_constants* c, float damp_rate_body1,
-                            float damp_rate_body2, gi_stream_t stream) {
-    if (n < 0 || !c || (n > 0 && (!pos || !vel || !force)))
-        return set_error(GI_E_INVALID_ARG, "gi_move_damped_separate: invalid arguments");
+                            const gi_move_separate_constants* c, float damp_rate_body1, float damp_rate_body2,
+                            gi_stream_t stream) {
+    int rc = check_particles("gi_move_damped_separate", pos, vel, force, n);
+    if (rc) return rc;
+    if (!c) return set_error(GI_E_INVALID_ARG, "gi_move_damped_separate: null constants");
     if (n == 0) return 0;
     hipLaunchKernelGGL(k_move_damped_separate, dim3(nblk(n, 256)), dim3(256), 0, as_stream(stream),
                        (float4*)pos, (float4*)vel, (const float4*)force, n, c->Dt, c->DtOverMassFe,
-                       c->DtOverMassSi, c->boarder1, c->boarder2, c->boarder3, damp_rate_body1,
-                       damp_rate_body2);
+                       c->DtOverMassSi, c->boarder1, c->boarder2, c->boarder3, damp_rate_body1, damp_rate_body2);
     GI_HIP(hipGetLastError());
     return 0;
 }
+
+int gi_center_of_mass_velocity(const gi_float4* pos, const gi_float4* vel, int n, int nfe, double mass_fe,
+                               double mass_si, double* out6, gi_stream_t stream) {
+    if (n <= 0 || !pos || !vel || !out6 || nfe < 0 || nfe > n)
+        return set_error(GI_E_INVALID_ARG, "gi_center_of_mass_velocity: invalid arguments");
+    DevCtx* ctx;
+    int rc = current_ctx(&ctx);
+    if (rc) return rc;
+    hipStream_t st = as_stream(stream);
+    const int nb = nblk(n, 256);
+    if ((rc = ctx->rc_blocks.ensure((size_t)nb * kMoments1 + 1))) return rc;
+    if ((rc = ctx->rc_sums.ensure(kMoments1 + 1))) return rc;
+    hipLaunchKernelGGL(k_moments_one, dim3(nb), dim3(256), 0, st, (const float4*)pos, (const float4*)vel, n, nfe,
+                       mass_fe, mass_si, ctx->rc_blocks.p);
+    hipLaunchKernelGGL(k_moments_one_final, dim3(1), dim3(64), 0, st, (const double*)ctx->rc_blocks.p, nb,
+                       ctx->rc_sums.p);
+    GI_HIP(hipGetLastError());
+    double s[kMoments1];
+    GI_HIP(hipMemcpyAsync(s, ctx->rc_sums.p, sizeof s, hipMemcpyDeviceToHost, st));
+    GI_HIP(hipStreamSynchronize(st));
+    for (int k = 0; k < 6; ++k) out6[k] = s[k + 1] / s[0];
+    return 0;
+}
+
+int gi_recenter(gi_float4* pos, gi_float4* vel, int n, int nfe, double mass_fe, double mass_si,
+                const double* target_com3, const double* target_vel3, gi_stream_t stream) {
+    if (n <= 0 || !pos || !vel || !target_com3 || !target_vel3 || nfe < 0 || nfe > n)
+        return set_error(GI_E_INVALID_ARG, "gi_recenter: invalid arguments");
+    DevCtx* ctx;
+    int rc = current_ctx(&ctx);
+    if (rc) return rc;
+    return launch_recenter(ctx, (float4*)pos, (float4*)vel, n, nfe, mass_fe, mass_si, target_com3, target_vel3,
+                           as_stream(stream));
+}
+
+int gi_conserved_quantities(const gi_float4* pos, const gi_float4* vel, int n, const gi_force_collision_constants* c,
+                            double* out9, gi_stream_t stream) {
+    if (n <= 0 || !pos || !vel || !c || !out9 || c->NFe < 0 || c->NFe > n)
+        return set_error(GI_E_INVALID_ARG, "gi_conserved_quantities: invalid arguments");
+    DevCtx* ctx;
+    int rc = current_ctx(&ctx);
+    if (rc) return rc;
+    hipStream_t st = as_stream(stream);
+    std::vector<SimDesc> sims{make_sim(0, c->NFe, n, law_of(c))};
+    if ((rc = ensure_plan(&ctx->ws, sims, st))) return rc;
+    if ((rc = run_conserved(ctx, &ctx->ws, (const float4*)pos, (const float4*)vel, st))) return rc;
+    double m[kMoments];
+    GI_HIP(hipMemcpyAsync(m, ctx->ws.sums.p, sizeof m, hipMemcpyDeviceToHost, st));
+    GI_HIP(hipStreamSynchronize(st));
+    moments_to_conserved(m, out9);
+    return 0;
+}
+
+int gi_impact_steps(gi_float4* pos, gi_float4* vel, gi_float4* force, int n, const gi_force_collision_constants* fc,
+                    const gi_move_collision_constants* mc, int n_steps, int recenter_rate, const double* target_com3,
+                    const double* target_vel3, gi_stream_t stream) {
+    int rc = check_particles("gi_impact_steps", pos, vel, force, n);
+    if (rc) return rc;
+    if (!fc || !mc || n_steps < 0 || recenter_rate < 0 || fc->NFe < 0 || fc->NFe > n || mc->NFe != fc->NFe)
+        return set_error(GI_E_INVALID_ARG, "gi_impact_steps: invalid arguments");
+    if (recenter_rate > 0 && (!target_com3 || !target_vel3))
+        return set_error(GI_E_INVALID_ARG, "gi_impact_steps: recenter needs target_com3 and target_vel3");
+    if (n == 0 || n_steps == 0) return 0;
+    DevCtx* ctx;
+    if ((rc = current_ctx(&ctx))) return rc;
+    StreamBridge br;
+    if ((rc = br.begin(ctx, as_stream(stream)))) return rc;
+    std::vector<SimDesc> sims{make_sim(0, fc->NFe, n, law_of(fc))};
+    if ((rc = ensure_plan(&ctx->ws, sims, br.run))) return rc;
+    // Make sure the plan uploads are complete before they are captured around.
+    GI_HIP(hipStreamSynchronize(br.run));
+    float4 *P = (float4*)pos, *V = (float4*)vel, *F = (float4*)force;
+    const gi_move_collision_constants m = *mc;
+    std::vector<unsigned char> key;
+    key_append(key, 1);
+    key_append(key, P);
+    key_append(key, V);
+    key_append(key, F);
+    key_append(key, n);
+    key_append(key, *fc);
+    key_append(key, m);
+    hipGraphExec_t step;
+    rc = get_graph(ctx, ctx->graphs, key, ctx->ws.generation,
+                   [&](hipStream_t cs) -> int {
+                       int r = run_force(ctx, &ctx->ws, P, V, F, cs, false);
+                       if (r) return r;
+                       hipLaunchKernelGGL(k_move_collision, dim3(nblk(n, 256)), dim3(256), 0, cs, P, V,
+                                          (const float4*)F, n, m.Dt, m.DtOverMassFe, m.DtOverMassSi, m.NFe);
+                       GI_HIP(hipGetLastError());
+                       return 0;
+                   },
+                   &step);
+    if (rc) return rc;
+    hipGraphExec_t rec = nullptr;
+    if (recenter_rate > 0) {
+        // The recenter uses the move's masses: m = Dt / (Dt/m).
+        const double mfe = (double)m.Dt / (double)m.DtOverMassFe;
+        const double msi = (double)m.Dt / (double)m.DtOverMassSi;
+        std::vector<unsigned char> k2;
+        key_append(k2, 2);
+        key_append(k2, P);
+        key_append(k2, V);
+        key_append(k2, n);
+        key_append(k2, fc->NFe);
+        for (int a = 0; a < 3; ++a) {
+            key_append(k2, target_com3[a]);
+            key_append(k2, target_vel3[a]);
+        }
+        // make sure the recenter scratch exists before capture
+        if ((rc = ctx->rc_blocks.ensure((size_t)nblk(n, 256) * kMoments1 + 1))) return rc;
+        if ((rc = ctx->rc_sums.ensure(kMoments1 + 1))) return rc;
+        rc = get_graph(ctx, ctx->graphs, k2, ctx->ws.generation,
+                       [&](hipStream_t cs) -> int {
+                           return launch_recenter(ctx, P, V, n, fc->NFe, mfe, msi, target_com3, target_vel3, cs);
+                       },
+                       &rec);
+        if (rc) return rc;
+    }
+    for (int s = 1; s <= n_steps; ++s) {
+        GI_HIP(hipGraphLaunch(step, br.run));
+        if (rec && s % recenter_rate == 0) GI_HIP(hipGraphLaunch(rec, br.run));
+    }
+    return br.end();
+}
+
+// ---- ensemble ---------------------------------------------------------------
+
+int gi_ensemble_create(gi_ensemble** out, int n_sims, const int* n, const gi_force_collision_constants* fc,
+                       const gi_move_collision_constants* mc, gi_float4* pos, gi_float4* vel, gi_stream_t stream) {
+    if (!out || n_sims <= 0 || !n || !fc || !mc || !pos || !vel)
+        return set_error(GI_E_INVALID_ARG, "gi_ensemble_create: invalid arguments");
+    *out = nullptr;
+    int dev = 0;
+    GI_HIP(hipGetDevice(&dev));
+    gi_ensemble* e = new gi_ensemble();
+    e->device = dev;
+    int off = 0;
+    for (int s = 0; s < n_sims; ++s) {
+        if (n[s] <= 0 || fc[s].NFe < 0 || fc[s].NFe > n[s] || mc[s].NFe != fc[s].NFe) {
+            delete e;
+            return set_error(GI_E_INVALID_ARG, "gi_ensemble_create: simulation %d has invalid n/NFe", s);
+        }
+        SimDesc d = make_sim(off, off + fc[s].NFe, off + n[s], law_of(&fc[s]));
+        d.dt = mc[s].Dt;
+        d.dt_over_mass_fe = mc[s].DtOverMassFe;
+        d.dt_over_mass_si = mc[s].DtOverMassSi;
+        e->sims.push_back(d);
+        off += n[s];
+        if (off > (1 << 21)) {
+            delete e;
+            return set_error(GI_E_TOO_LARGE, "gi_ensemble_create: more than 2^21 particles");
+        }
+    }
+    e->pos = (float4*)pos;
+    e->vel = (float4*)vel;
+    e->stream = as_stream(stream);
+    int rc = e->force.ensure((size_t)off);
+    if (!rc) rc = ensure_plan(&e->ws, e->sims, e->stream);
+    if (rc) {
+        delete e;
+        return rc;
+    }
+    *out = e;
+    return 0;
+}
+
+int gi_ensemble_destroy(gi_ensemble* e) {
+    delete e;
+    return 0;
+}
+
+int gi_ensemble_set_active(gi_ensemble* e, const int* active) {
+    if (!e || !active) return set_error(GI_E_INVALID_ARG, "gi_ensemble_set_active: invalid arguments");
+    for (size_t s = 0; s < e->sims.size(); ++s) e->sims[s].active = active[s] ? 1 : 0;
+    return 0;
+}
+
+int gi_ensemble_steps(gi_ensemble* e, int n_steps) {
+    if (!e || n_steps < 0) return set_error(GI_E_INVALID_ARG, "gi_ensemble_steps: invalid arguments");
+    if (n_steps == 0) return 0;
+    DevCtx* ctx;
+    int rc = current_ctx(&ctx);
+    if (rc) return rc;
+    if (ctx->device != e->device) return set_error(GI_E_STATE, "gi_ensemble_steps: wrong current device");
+    StreamBridge br;
+    if ((rc = br.begin(ctx, e->stream))) return rc;
+    const unsigned long long gen0 = e->ws.generation;
+    if ((rc = ensure_plan(&e->ws, e->sims, br.run))) return rc;
+    if (e->ws.generation != gen0) GI_HIP(hipStreamSynchronize(br.run));
+    const int ntot = e->ws.plan.ntot;
+    const int ns = (int)e->ws.plan.sims.size();
+    std::vector<unsigned char> key;
+    key_append(key, 3);
+    hipGraphExec_t step;
+    rc = get_graph(ctx, e->graphs, key, e->ws.generation,
+                   [&](hipStream_t cs) -> int {
+                       int r = run_force(ctx, &e->ws, e->pos, e->vel, e->force.p, cs, false);
+                       if (r) return r;
+                       hipLaunchKernelGGL(k_move, dim3(nblk(ntot, 256)), dim3(256), 0, cs, e->pos, e->vel,
+                                          (const float4*)e->force.p, ntot, e->ws.sims.p, ns);
+                       GI_HIP(hipGetLastError());
+                       return 0;
+                   },
+                   &step);
+    if (rc) return rc;
+    for (int s = 0; s < n_steps; ++s) GI_HIP(hipGraphLaunch(step, br.run));
+    return br.end();
+}
+
+int gi_ensemble_conserved(gi_ensemble* e, double* out) {
+    if (!e || !out) return set_error(GI_E_INVALID_ARG, "gi_ensemble_conserved: invalid arguments");
+    DevCtx* ctx;
+    int rc = current_ctx(&ctx);
+    if (rc) return rc;
+    std::vector<SimDesc> all = e->sims;
+    for (auto& s : all) s.active = 1;
+    StreamBridge br;
+    if ((rc = br.begin(ctx, e->stream))) return rc;
+    if ((rc = ensure_plan(&e->ws_diag, all, br.run))) return rc;
+    if ((rc = run_conserved(ctx, &e->ws_diag, e->pos, e->vel, br.run))) return rc;
+    const size_t ns = all.size();
+    std::vector<double> m(ns * kMoments);
+    GI_HIP(hipMemcpyAsync(m.data(), e->ws_diag.sums.p, m.size() * sizeof(double), hipMemcpyDeviceToHost, br.run));
+    GI_HIP(hipStreamSynchronize(br.run));
+    for (size_t s = 0; s < ns; ++s) moments_to_conserved(&m[s * kMoments], out + 9 * s);
+    return br.end();
+}
+
+int gi_ensemble_work(gi_ensemble* e, long long* particles, double* pairs) {
+    if (!e) return set_error(GI_E_INVALID_ARG, "gi_ensemble_work: null ensemble");
+    long long p = 0;
+    double q = 0.0;
+    for (auto& s : e->sims)
+        if (s.active) {
+            p += s.end - s.start;
+            q += (double)(s.end - s.start) * (double)(s.end - s.start);
+        }
+    if (particles) *particles = p;
+    if (pairs) *pairs = q;
+    return 0;
+}
+
+// ---- timing -------------------------------------------------------------------
 
 int gi_set_kernel_timing(int enable) {
     g_timing = enable != 0;
@@ -524,8 +987,11 @@ int gi_set_kernel_timing(int enable) {
 }
 
 int gi_kernel_timing(const char* kernel, long long* launches, double* total_ms) {
-    if (kernel && std::strcmp(kernel, "gravity") != 0)
-        return set_error(GI_E_INVALID_ARG, "unknown kernel '%s' (timed kernel: gravity)", kernel);
+    int kind = 0;
+    if (kernel && std::strcmp(kernel, "move") == 0)
+        kind = 1;
+    else if (kernel && std::strcmp(kernel, "gravity") != 0)
+        return set_error(GI_E_INVALID_ARG, "unknown kernel '%s' (timed kernels: gravity, move)", kernel);
     DevCtx* ctx;
     int rc = current_ctx(&ctx);
     if (rc) return rc;
@@ -533,25 +999,27 @@ int gi_kernel_timing(const char* kernel, long long* launches, double* total_ms) 
         GI_HIP(hipEventSynchronize(tp.b));
         float ms = 0.0f;
         GI_HIP(hipEventElapsedTime(&ms, tp.a, tp.b));
-        ctx->timed_ms += ms;
-        ctx->timed_launches++;
+        ctx->timed_ms[tp.kind] += ms;
+        ctx->timed_launches[tp.kind]++;
         ctx->pool.push_back(tp);
     }
     ctx->pending.clear();
-    if (launches) *launches = ctx->timed_launches;
-    if (total_ms) *total_ms = ctx->timed_ms;
+    if (launches) *launches = ctx->timed_launches[kind];
+    if (total_ms) *total_ms = ctx->timed_ms[kind];
     return 0;
 }
 
 int gi_reset_kernel_timing(void) {
-    DevCtx* ctx;
-    int rc = current_ctx(&ctx);
-    if (rc) return rc;
     long long l;
     double t;
-    if ((rc = gi_kernel_timing(nullptr, &l, &t))) return rc;
-    ctx->timed_launches = 0;
-    ctx->timed_ms = 0.0;
+    int rc = gi_kernel_timing(nullptr, &l, &t);
+    if (rc) return rc;
+    DevCtx* ctx;
+    if ((rc = current_ctx(&ctx))) return rc;
+    for (int k = 0; k < kTimedKinds; ++k) {
+        ctx->timed_launches[k] = 0;
+        ctx->timed_ms[k] = 0.0;
+    }
     return 0;
 }
 

@@ -278,7 +278,10 @@ __global__ __launch_bounds__(256) void k_finish(const float4* __restrict__ pos,
     const float4 vi = vel[i];
     const int ccx = cell_coord(pi.x), ccy = cell_coord(pi.y), ccz = cell_coord(pi.z);
     const bool ife = i < sd.fe_end;
-    float ex = 0.0f, ey = 0.0f, ez = 0.0f;
+    // Contact terms are rounded exactly as the reference rounds them (float
+    // products of a float magnitude); only their sum is carried in FP64, so
+    // cancellation between large contact forces adds no summation error.
+    double ex = 0.0, ey = 0.0, ez = 0.0;
     for (int oz = -1; oz <= 1; ++oz)
         for (int oy = -1; oy <= 1; ++oy)
             for (int ox = -1; ox <= 1; ++ox) {
@@ -306,14 +309,14 @@ __global__ __launch_bounds__(256) void k_finish(const float4* __restrict__ pos,
                     // What pass 1 added for this pair.
                     const float g = gravity_product(sd.law, test);
                     const float sf = g * fast_inv_r3(fast_r2(dpx, dpy, dpz));
-                    ex += fm * dpx * invr - sf * dpx;
-                    ey += fm * dpy * invr - sf * dpy;
-                    ez += fm * dpz * invr - sf * dpz;
+                    ex += (double)(fm * dpx * invr) - (double)(sf * dpx);
+                    ey += (double)(fm * dpy * invr) - (double)(sf * dpy);
+                    ez += (double)(fm * dpz * invr) - (double)(sf * dpz);
                 }
             }
-    force[i].x = fx + ex;
-    force[i].y = fy + ey;
-    force[i].z = fz + ez;
+    force[i].x = (float)((double)fx + ex);
+    force[i].y = (float)((double)fy + ey);
+    force[i].z = (float)((double)fz + ez);
 }
 
 // ---------------------------------------------------------------------------
@@ -523,18 +526,67 @@ __global__ void k_moments_final(const double* __restrict__ blocks, const int* __
     for (int k = 0; k < kMoments; ++k) sums[s * kMoments + k] = acc[k];
 }
 
-// Recenter shift: positions += target_com - com, velocities += target_vel - vbar.
+// FP64 mass moments of particles [0, n) (iron for i < nfe): per block
+// {M, M x, M y, M z, M vx, M vy, M vz}, summed by k_moments_one_final.
+constexpr int kMoments1 = 7;
+
+__global__ __launch_bounds__(256) void k_moments_one(const float4* __restrict__ pos,
+                                                     const float4* __restrict__ vel, int n, int nfe,
+                                                     double mass_fe, double mass_si,
+                                                     double* __restrict__ out) {
+    __shared__ double sh[kMoments1][256];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    double v[kMoments1];
+#pragma unroll
+    for (int k = 0; k < kMoments1; ++k) v[k] = 0.0;
+    if (i < n) {
+        const double m = i < nfe ? mass_fe : mass_si;
+        const float4 p = pos[i];
+        const float4 u = vel[i];
+        v[0] = m;
+        v[1] = m * p.x;
+        v[2] = m * p.y;
+        v[3] = m * p.z;
+        v[4] = m * u.x;
+        v[5] = m * u.y;
+        v[6] = m * u.z;
+    }
+#pragma unroll
+    for (int k = 0; k < kMoments1; ++k) sh[k][threadIdx.x] = v[k];
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < off) {
+#pragma unroll
+            for (int k = 0; k < kMoments1; ++k) sh[k][threadIdx.x] += sh[k][threadIdx.x + off];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < kMoments1) out[blockIdx.x * kMoments1 + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+// One block: column k summed over the nb block rows in block order.
+__global__ void k_moments_one_final(const double* __restrict__ blocks, int nb, double* __restrict__ sums) {
+    const int k = threadIdx.x;
+    if (k >= kMoments1) return;
+    double acc = 0.0;
+    for (int b = 0; b < nb; ++b) acc += blocks[b * kMoments1 + k];
+    sums[k] = acc;
+}
+
+// Recenter (impactAddTime.cu:433-455) with the means taken in FP64 on the
+// device: positions += target_com - com, velocities += target_vel - mean_vel.
 __global__ void k_recenter_apply(float4* __restrict__ pos, float4* __restrict__ vel, int n,
-                                 const double* __restrict__ sums, const double* __restrict__ target6) {
+                                 const double* __restrict__ sums, double tcx, double tcy, double tcz,
+                                 double tvx, double tvy, double tvz) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const double M = sums[0];
-    const float sx = (float)(target6[0] - sums[1] / M);
-    const float sy = (float)(target6[1] - sums[2] / M);
-    const float sz = (float)(target6[2] - sums[3] / M);
-    const float ux = (float)(target6[3] - sums[4] / M);
-    const float uy = (float)(target6[4] - sums[5] / M);
-    const float uz = (float)(target6[5] - sums[6] / M);
+    const float sx = (float)(tcx - sums[1] / M);
+    const float sy = (float)(tcy - sums[2] / M);
+    const float sz = (float)(tcz - sums[3] / M);
+    const float ux = (float)(tvx - sums[4] / M);
+    const float uy = (float)(tvy - sums[5] / M);
+    const float uz = (float)(tvz - sums[6] / M);
     float4 p = pos[i];
     float4 v = vel[i];
     p.x += sx;
@@ -551,6 +603,8 @@ __global__ void k_recenter_apply(float4* __restrict__ pos, float4* __restrict__ 
 template __global__ void k_gravity<4, 128, false>(const float4*, const GravTask*, int, const SimDesc*, float4*);
 template __global__ void k_gravity<8, 128, false>(const float4*, const GravTask*, int, const SimDesc*, float4*);
 template __global__ void k_gravity<2, 128, false>(const float4*, const GravTask*, int, const SimDesc*, float4*);
+template __global__ void k_gravity<2, 128, true>(const float4*, const GravTask*, int, const SimDesc*, float4*);
 template __global__ void k_gravity<4, 128, true>(const float4*, const GravTask*, int, const SimDesc*, float4*);
+template __global__ void k_gravity<8, 128, true>(const float4*, const GravTask*, int, const SimDesc*, float4*);
 
 }  // namespace gi

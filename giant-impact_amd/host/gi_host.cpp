@@ -1,6 +1,6 @@
 // gi_host.cpp -- units, kernel constants, parameter files and seeded
 // synthetic bodies for the Giant-Impact step loop (host side, C ABI).
-#include "gi_host.h"
+#include "../../include/giant_impact_host.h"
 
 #include <algorithm>
 #include <cmath>

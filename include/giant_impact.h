@@ -236,9 +236,19 @@ int gi_ensemble_conserved(gi_ensemble *e, double *out);
 /* Total particles and total pair interactions of one step of the active set. */
 int gi_ensemble_work(gi_ensemble *e, long long *particles, double *pairs);
 
+/* ---- tiling of the all-pairs pass (the autotuner's control) -------------- */
+
+/* Override the tiling chosen from the built-in tuning table: r = i-particles
+ * per lane (2, 4 or 8), chunk_len = j-particles per task (0 = derived from
+ * target_tasks), target_tasks = wavefront tasks to aim for, min_chunk = lower
+ * bound of chunk_len.  r = 0 restores the built-in table.  Plans built with
+ * another tiling are rebuilt on their next use. */
+int gi_set_tiling(int r, long long chunk_len, long long target_tasks, long long min_chunk);
+
 /* ---- kernel timing (HIP events around the all-pairs force kernel) ------- */
 
-/* Enable/disable per-launch timing of the all-pairs gravity kernel; when on,
+/* Enable/disable per-launch timing (HIP events on the launching stream) of the
+ * all-pairs gravity kernel ("gravity") and of gi_move_collision ("move"); when on,
  * gi_kernel_timing returns the launch count and the summed milliseconds since
  * the last reset. */
 int gi_set_kernel_timing(int enable);

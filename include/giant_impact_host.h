@@ -5,7 +5,7 @@
 #ifndef GI_HOST_H
 #define GI_HOST_H
 
-#include "../../include/giant_impact.h"
+#include "giant_impact.h"
 
 #ifdef __cplusplus
 extern "C" {

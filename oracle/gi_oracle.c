@@ -112,11 +112,11 @@ int or_units_compute(or_units *u)
     u->mass_fe = MassFe;
     u->mass_si = MassSi;
     u->diameter = 1.0;
-    /* :381-382, :394-395 */
+    /* :381-382, :394-395 ("KFe *= a*b*c/d" evaluates the right side first) */
     u->mass_of_body1 = massOfBody1 / unitMass;
     u->mass_of_body2 = massOfBody2 / unitMass;
-    u->k_fe_sim = u->k_fe * unitTime * unitTime * unitLength / unitMass;
-    u->k_si_sim = u->k_si * unitTime * unitTime * unitLength / unitMass;
+    u->k_fe_sim = u->k_fe * (unitTime * unitTime * unitLength / unitMass);
+    u->k_si_sim = u->k_si * (unitTime * unitTime * unitLength / unitMass);
     return 0;
 }
 
@@ -474,10 +474,13 @@ static inline double pair_mag_f64(const pair_law64 *L, int test, double r, doubl
 
 static void force_f64_generic(const or_float4 *pos, const or_float4 *vel, double *force3,
                               double *cond, int *near, double margin, int n, const pair_law64 *L,
-                              int separate, int b1, int b2, int b3, int nfe, int threads)
+                              int separate, int b1, int b2, int b3, int nfe, int i_begin,
+                              int i_end, int threads)
 {
+    if (i_begin < 0) i_begin = 0;
+    if (i_end > n) i_end = n;
 #pragma omp parallel for schedule(dynamic, 16) num_threads(OR_OMP_THREADS(threads))
-    for (int id = 0; id < n; id++) {
+    for (int id = i_begin; id < i_end; id++) {
         double fx = 0, fy = 0, fz = 0, cs = 0;
         int nr = 0;
         int j0 = 0, j1 = n, sw = nfe;
@@ -519,13 +522,14 @@ static void force_f64_generic(const or_float4 *pos, const or_float4 *vel, double
 
 void or_force_collision_f64(const or_float4 *pos, const or_float4 *vel, double *force3,
                             double *cond, int *near, double rel_margin, int n,
-                            const or_force_collision_constants *c, int threads)
+                            const or_force_collision_constants *c, int i_begin, int i_end,
+                            int threads)
 {
     pair_law64 L = law_from(c->GMassFeFe, c->GMassFeSi, c->KFeFe, c->KSiSi, c->KFeSi, c->KRFeFe,
                             c->KRSiSi, c->KRFeSi, c->KRMix, c->ShellBreakFe, c->ShellBreakSi,
                             c->ShellBreakFeSi1, c->ShellBreakFeSi2);
     force_f64_generic(pos, vel, force3, cond, near, rel_margin, n, &L, 0, 0, 0, 0, c->NFe,
-                      threads);
+                      i_begin, i_end, threads);
 }
 
 void or_force_separate_f64(const or_float4 *pos, const or_float4 *vel, double *force3,
@@ -536,7 +540,7 @@ void or_force_separate_f64(const or_float4 *pos, const or_float4 *vel, double *f
                             c->KRSiSi, c->KRFeSi, c->KRMix, c->ShellBreakFe, c->ShellBreakSi,
                             c->ShellBreakFeSi1, c->ShellBreakFeSi2);
     force_f64_generic(pos, vel, force3, cond, near, rel_margin, n, &L, 1, c->boarder1,
-                      c->boarder2, c->boarder3, 0, threads);
+                      c->boarder2, c->boarder3, 0, 0, n, threads);
 }
 
 /* Conserved quantities in FP64.  The pair potential integrates the approach

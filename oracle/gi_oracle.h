@@ -86,12 +86,14 @@ void or_move_damped_separate_f32(or_float4 *pos, or_float4 *vel, const or_float4
                                  const or_move_separate_constants *c, float damp1, float damp2);
 
 /* ---- FP64 twins (truth) ---- */
-/* force3: 3n doubles.  cond: n doubles = sum_j |f_ij| (the conditioning
+/* Outputs are written for i in [i_begin, i_end) only (the separate variant:
+ * all i).  force3: 3n doubles.  cond: n doubles = sum_j |f_ij| (the conditioning
  * denominator), may be NULL.  near: n ints, set to 1 when some pair's branch
  * decision of particle i lies within rel_margin of a threshold, may be NULL. */
 void or_force_collision_f64(const or_float4 *pos, const or_float4 *vel, double *force3,
                             double *cond, int *near, double rel_margin, int n,
-                            const or_force_collision_constants *c, int threads);
+                            const or_force_collision_constants *c, int i_begin, int i_end,
+                            int threads);
 void or_force_separate_f64(const or_float4 *pos, const or_float4 *vel, double *force3,
                            double *cond, int *near, double rel_margin, int n,
                            const or_force_separate_constants *c, int threads);

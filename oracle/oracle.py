@@ -87,7 +87,7 @@ def load() -> C.CDLL:
         "or_move_collision_f32": (None, [vp, vp, vp, i, P(MC)]),
         "or_force_separate_f32": (None, [vp, vp, vp, i, P(FS), i]),
         "or_move_damped_separate_f32": (None, [vp, vp, vp, i, P(MS), C.c_float, C.c_float]),
-        "or_force_collision_f64": (None, [vp, vp, vp, vp, vp, d, i, P(FC), i]),
+        "or_force_collision_f64": (None, [vp, vp, vp, vp, vp, d, i, P(FC), i, i, i]),
         "or_force_separate_f64": (None, [vp, vp, vp, vp, vp, d, i, P(FS), i]),
         "or_conserved_f64": (None, [vp, vp, i, P(FC), vp, i]),
         "or_com_vel_collision_f32": (None, [vp, vp, i, i, d, d, vp]),
@@ -178,15 +178,18 @@ def move_damped_separate_f32(pos, vel, force, ms, d1, d2):
     return pos, vel
 
 
-def force_collision_f64(pos, vel, fc, margin=1e-6, nthreads=None):
-    """FP64 truth: (force (n,3), cond (n,) = sum_j |f_ij|, near (n,) bool)."""
+def force_collision_f64(pos, vel, fc, margin=1e-6, nthreads=None, i_begin=0, i_end=None):
+    """FP64 truth: (force (n,3), cond (n,) = sum_j |f_ij|, near (n,) bool).
+
+    Only rows [i_begin, i_end) are computed; the other rows are zero."""
     pos, vel = _f4(pos), _f4(vel)
     n = pos.shape[0]
     f = np.zeros((n, 3), dtype=np.float64)
     cond = np.zeros(n, dtype=np.float64)
     near = np.zeros(n, dtype=np.int32)
     load().or_force_collision_f64(
-        _ptr(pos), _ptr(vel), _ptr(f), _ptr(cond), _ptr(near), margin, n, C.byref(as_fc(fc)), int(nthreads or threads())
+        _ptr(pos), _ptr(vel), _ptr(f), _ptr(cond), _ptr(near), margin, n, C.byref(as_fc(fc)),
+        int(i_begin), int(n if i_end is None else i_end), int(nthreads or threads())
     )
     return f, cond, near.astype(bool)
 

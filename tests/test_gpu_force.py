@@ -1,0 +1,242 @@
+"""GPU parity of the four step-loop entry points against the CPU oracle.
+
+All calls go through the C ABI of libgiant_impact.so (gi_amd is a ctypes
+shim).  Tolerances are in tests/parity.py.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from conftest import f4
+from parity import assert_force_parity, force_metrics
+
+pytestmark = pytest.mark.gpu
+
+
+def _upload(gi, pos, vel):
+    n = len(pos)
+    return gi.DeviceArray(n, pos), gi.DeviceArray(n, vel), gi.DeviceArray(n, f4(n))
+
+
+def _gpu_force(gi, pos, vel, fc):
+    P, V, F = _upload(gi, pos, vel)
+    gi.force_collision(P, V, F, len(pos), fc)
+    gi.sync()
+    return F.download()
+
+
+CASES = {
+    # name: (n_target, n_impactor, seed, angle, speed, target spin)
+    "impact_2048": (1536, 512, 20261202, 45.0, 1.2, 0.0),
+    "ragged_1000_spin": (800, 200, 7, 30.0, 1.0, 0.5),
+    "ragged_4097": (3000, 1097, 3, 60.0, 1.4, 0.3),
+}
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_force_collision_parity(gpu, oracle, case):
+    nt, ni, seed, ang, spd, spin = CASES[case]
+    pos, vel, fc, _, _ = gpu.make_impact(nt, ni, seed=seed, angle_deg=ang, speed_vesc=spd, target_spin=spin)
+    print(f"case={case} n={nt + ni}", flush=True)
+    F = _gpu_force(gpu, pos, vel, fc)
+    F64, cond, near = oracle.force_collision_f64(pos, vel, fc)
+    Fref = oracle.force_collision_f32(pos, vel, fc)
+    met = force_metrics(F, F64, cond, near, Fref)
+    print(met)
+    assert_force_parity(met)
+    assert np.all(F[:, 3] == 0.0)  # .w untouched (was zero)
+
+
+def test_force_collision_during_impact(gpu, oracle):
+    """After 400 steps the bodies are in contact: receding branches, deep
+    overlaps and Fe-Si pairs all occur."""
+    pos, vel, fc, mc, _ = gpu.make_impact(1536, 512, seed=20261202, angle_deg=45.0, speed_vesc=1.2,
+                                          gap_in_target_radii=0.3)
+    n = len(pos)
+    P, V, F = _upload(gpu, pos, vel)
+    gpu.call("gi_impact_steps", P.ptr, V.ptr, F.ptr, n, C.byref(fc), C.byref(mc), 400, 0, None, None, None)
+    gpu.sync()
+    p, v = P.download(), V.download()
+    gpu.force_collision(P, V, F, n, fc)
+    gpu.sync()
+    Fg = F.download()
+    F64, cond, near = oracle.force_collision_f64(p, v, fc)
+    met = force_metrics(Fg, F64, cond, near, oracle.force_collision_f32(p, v, fc))
+    print(met)
+    # the state really has receding deep contacts
+    d = p[None, :, :3] - p[:, None, :3]
+    r = np.sqrt((d**2).sum(-1))
+    dot = (d * (v[None, :, :3] - v[:, None, :3])).sum(-1)
+    assert np.sum((r < 0.98) & (dot >= 1.0)) > 0
+    assert_force_parity(met)
+
+
+def _generator_bodies(gi, n1, n2, seed):
+    """Two bodies in the generator layout [Fe1, Si1, Fe2, Si2], both centred at
+    the origin (so the cross-body pairs overlap in space and must be skipped)."""
+    pos_t, vel_t, fc_t, _, it = gi.make_impact(n1, 0, seed=seed, target_spin=0.4)
+    pos_i, vel_i, fc_i, _, ii = gi.make_impact(n2, 0, seed=seed + 1, target_spin=0.2)
+    rng = np.random.default_rng(seed)
+    vel_t[:, :3] += rng.normal(0, 2.0, (n1, 3)).astype(np.float32)
+    vel_i[:, :3] += rng.normal(0, 2.0, (n2, 3)).astype(np.float32)
+    pos = np.concatenate([pos_t, pos_i])
+    vel = np.concatenate([vel_t, vel_i])
+    u = gi.units(gi.generator_setup_defaults(n1 + n2))
+    fs, ms = gi.separate_constants(u)
+    fs.boarder1, fs.boarder2, fs.boarder3 = it.nfe1, n1, n1 + ii.nfe1
+    ms.boarder1, ms.boarder2, ms.boarder3 = fs.boarder1, fs.boarder2, fs.boarder3
+    return pos, vel, fs, ms
+
+
+def test_force_separate_parity(gpu, oracle):
+    pos, vel, fs, _ = _generator_bodies(gpu, 1200, 500, 21)
+    n = len(pos)
+    P, V, F = _upload(gpu, pos, vel)
+    gpu.force_separate(P, V, F, n, fs)
+    gpu.sync()
+    Fg = F.download()
+    F64, cond, near = oracle.force_separate_f64(pos, vel, fs)
+    met = force_metrics(Fg, F64, cond, near, oracle.force_separate_f32(pos, vel, fs))
+    print(met)
+    assert_force_parity(met)
+
+
+def test_move_kernels_are_bitwise_the_reference(gpu, oracle):
+    pos, vel, fc, mc, _ = gpu.make_impact(900, 300, seed=4)
+    n = len(pos)
+    rng = np.random.default_rng(0)
+    frc = f4(n)
+    frc[:, :3] = rng.normal(0, 300, (n, 3))
+    frc[:, 3] = 5.0
+    P, V, F = gpu.DeviceArray(n, pos), gpu.DeviceArray(n, vel), gpu.DeviceArray(n, frc)
+    gpu.move_collision(P, V, F, n, mc)
+    gpu.sync()
+    po, vo = oracle.move_collision_f32(pos, vel, frc, mc)
+    assert np.array_equal(P.download().view(np.uint32), po.view(np.uint32))
+    assert np.array_equal(V.download().view(np.uint32), vo.view(np.uint32))
+    # damped generator move, both bodies, both materials
+    u = gpu.units(gpu.generator_setup_defaults(n))
+    _, ms = gpu.separate_constants(u)
+    ms.boarder1, ms.boarder2, ms.boarder3 = 100, 700, 850
+    P.upload(pos)
+    V.upload(vel)
+    gpu.move_damped_separate(P, V, F, n, ms, 8.0, 3.0)
+    gpu.sync()
+    po, vo = oracle.move_damped_separate_f32(pos, vel, frc, ms, 8.0, 3.0)
+    assert np.array_equal(P.download().view(np.uint32), po.view(np.uint32))
+    assert np.array_equal(V.download().view(np.uint32), vo.view(np.uint32))
+
+
+@pytest.mark.parametrize("n,nfe", [(1, 0), (1, 1), (2, 1), (3, 3), (65, 0), (257, 100), (640, 640)])
+def test_small_and_ragged_sizes(gpu, oracle, n, nfe):
+    rng = np.random.default_rng(n)
+    pos, vel = f4(n), f4(n)
+    pos[:, :3] = rng.uniform(-0.6, 0.6, (n, 3)) * max(1.0, n ** (1 / 3))
+    vel[:, :3] = rng.normal(0, 2, (n, 3))
+    fc = gpu.make_impact(10, 0)[2]
+    fc.NFe = nfe
+    print(f"n={n} nfe={nfe}", flush=True)
+    F = _gpu_force(gpu, pos, vel, fc)
+    F64, cond, near = oracle.force_collision_f64(pos, vel, fc)
+    if n == 1:
+        assert np.all(F[:, :3] == 0)
+        return
+    met = force_metrics(F, F64, cond, near, oracle.force_collision_f32(pos, vel, fc))
+    assert_force_parity(met, check_ref=False)
+
+
+def test_zero_particles_is_a_no_op(gpu):
+    fc, mc = gpu.impact_constants(gpu.units(gpu.generator_setup_defaults(1024)))
+    fc.NFe = mc.NFe = 0
+    gpu.force_collision(None, None, None, 0, fc)
+    gpu.move_collision(None, None, None, 0, mc)
+
+
+def test_large_n_sample_against_fp64(gpu, oracle):
+    """N = 65,536 (the single-impact size): 1,024 particles spread over all
+    materials checked against the FP64 twin, plus Newton's third law summed
+    over every particle."""
+    pos, vel, fc, _, _ = gpu.make_impact(52429, 13107, seed=2, angle_deg=45, speed_vesc=1.1, target_spin=0.5)
+    n = len(pos)
+    print(f"n={n}", flush=True)
+    F = _gpu_force(gpu, pos, vel, fc)
+    for a in (0, fc.NFe - 300, fc.NFe + 5000, n - 400):
+        F64, cond, near = oracle.force_collision_f64(pos, vel, fc, i_begin=a, i_end=a + 256)
+        sl = slice(a, a + 256)
+        met = force_metrics(F[sl], F64[sl], cond[sl], near[sl])
+        print(a, met)
+        assert_force_parity(met)
+    tot = F[:, :3].astype(np.float64).sum(0)
+    assert np.abs(tot).max() <= 1e-5 * np.abs(F[:, :3]).astype(np.float64).sum()
+
+
+def test_graph_stepper_equals_the_manual_loop(gpu):
+    pos, vel, fc, mc, _ = gpu.make_impact(1536, 512, seed=9)
+    n = len(pos)
+    P1, V1, F1 = _upload(gpu, pos, vel)
+    P2, V2, F2 = _upload(gpu, pos, vel)
+    for _ in range(12):
+        gpu.force_collision(P1, V1, F1, n, fc)
+        gpu.move_collision(P1, V1, F1, n, mc)
+    stream = C.c_void_p()
+    gpu.call("gi_stream_create", C.byref(stream))
+    gpu.call("gi_impact_steps", P2.ptr, V2.ptr, F2.ptr, n, C.byref(fc), C.byref(mc), 5, 0, None, None, stream)
+    gpu.call("gi_impact_steps", P2.ptr, V2.ptr, F2.ptr, n, C.byref(fc), C.byref(mc), 7, 0, None, None, stream)
+    gpu.call("gi_stream_sync", stream)
+    gpu.sync()
+    assert np.array_equal(P1.download(), P2.download())
+    assert np.array_equal(V1.download(), V2.download())
+    gpu.call("gi_stream_destroy", stream)
+
+
+def test_steps_track_the_reference_for_a_few_steps(gpu, oracle):
+    pos, vel, fc, mc, _ = gpu.make_impact(1536, 512, seed=20261202, angle_deg=45, speed_vesc=1.2)
+    n = len(pos)
+    P, V, F = _upload(gpu, pos, vel)
+    gpu.call("gi_impact_steps", P.ptr, V.ptr, F.ptr, n, C.byref(fc), C.byref(mc), 10, 0, None, None, None)
+    gpu.sync()
+    po, vo, _ = oracle.impact_steps_f32(pos, vel, fc, mc, 10)
+    p, v = P.download(), V.download()
+    dv = np.abs(v[:, :3] - vo[:, :3]).max()
+    assert dv <= 1e-4 * np.abs(vo[:, :3]).max()
+    assert np.abs(p[:, :3] - po[:, :3]).max() <= 1e-5 * np.abs(po[:, :3]).max()
+
+
+def test_device_recenter_and_moments(gpu, oracle):
+    pos, vel, fc, _, _ = gpu.make_impact(3000, 1000, seed=12)
+    pos[:, :3] += np.float32(7.5)
+    vel[:, :3] += np.float32(0.25)
+    n = len(pos)
+    P, V, _ = _upload(gpu, pos, vel)
+    out = (C.c_double * 6)()
+    mfe = float(fc.GMassFeSi)
+    gpu.call("gi_center_of_mass_velocity", P.ptr, V.ptr, n, fc.NFe, mfe, 1.0, out, None)
+    m = np.where(np.arange(n) < fc.NFe, mfe, 1.0)
+    com = (m[:, None] * pos[:, :3].astype(np.float64)).sum(0) / m.sum()
+    vm = (m[:, None] * vel[:, :3].astype(np.float64)).sum(0) / m.sum()
+    assert np.allclose(list(out)[:3], com, rtol=1e-12, atol=1e-9)
+    assert np.allclose(list(out)[3:], vm, rtol=1e-12, atol=1e-12)
+    tc = (C.c_double * 3)(1.0, -2.0, 0.5)
+    tv = (C.c_double * 3)(0.0, 0.1, 0.0)
+    gpu.call("gi_recenter", P.ptr, V.ptr, n, fc.NFe, mfe, 1.0, tc, tv, None)
+    gpu.call("gi_center_of_mass_velocity", P.ptr, V.ptr, n, fc.NFe, mfe, 1.0, out, None)
+    assert np.allclose(list(out)[:3], [1.0, -2.0, 0.5], atol=2e-6)
+    assert np.allclose(list(out)[3:], [0.0, 0.1, 0.0], atol=2e-7)
+
+
+def test_conserved_quantities_match_fp64_oracle(gpu, oracle):
+    pos, vel, fc, _, _ = gpu.make_impact(1536, 512, seed=5, target_spin=0.5)
+    n = len(pos)
+    P, V, _ = _upload(gpu, pos, vel)
+    out = (C.c_double * 9)()
+    gpu.call("gi_conserved_quantities", P.ptr, V.ptr, n, C.byref(fc), out, None)
+    g = np.array(list(out))
+    o = oracle.conserved_f64(pos, vel, fc)
+    print(g, o)
+    assert g[0] == pytest.approx(o[0], rel=1e-12)
+    assert g[1] == pytest.approx(o[1], rel=2e-6)
+    assert np.allclose(g[3:6], o[3:6], atol=1e-9 * abs(o[0]) ** 0.5 * n)
+    assert np.allclose(g[6:9], o[6:9], rtol=1e-9, atol=1e-9 * np.abs(o[6:9]).max())

@@ -1,0 +1,162 @@
+"""Host side of libgiant_impact.so on the CPU: units, constants, parameter
+files, synthetic bodies, argument validation and the exported ABI."""
+from __future__ import annotations
+
+import ctypes as C
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import REPO, f4
+
+GOLDEN = REPO / "tests" / "golden"
+
+
+def _bytes(s) -> bytes:
+    return bytes(memoryview(s))
+
+
+@pytest.mark.parametrize("n", [512, 2048, 8192, 65536, 262144, 524288])
+def test_units_and_constants_match_the_oracle(gi, oracle, n):
+    u = gi.units(gi.generator_setup_defaults(n))
+    o = oracle.units(n)
+    assert (u.nfe, u.nsi, u.nfe1, u.nsi1, u.nfe2, u.nsi2) == (o.nfe, o.nsi, o.nfe1, o.nsi1, o.nfe2, o.nsi2)
+    for a, b in [("unit_length", "unit_length"), ("unit_mass", "unit_mass"), ("unit_time", "unit_time"),
+                 ("mass_fe", "mass_fe"), ("k_fe", "k_fe_sim"), ("k_si", "k_si_sim"),
+                 ("mass_of_body1", "mass_of_body1")]:
+        assert getattr(u, a) == getattr(o, b), a
+    fc, mc = gi.impact_constants(u)
+    ofc, omc = oracle.impact_constants(o)
+    assert _bytes(fc) == _bytes(ofc) and _bytes(mc) == _bytes(omc)
+    fs, ms = gi.separate_constants(u)
+    ofs, oms = oracle.separate_constants(o)
+    assert _bytes(fs) == _bytes(ofs) and _bytes(ms) == _bytes(oms)
+
+
+def test_struct_sizes_match_the_reference_layout(gi):
+    assert C.sizeof(gi.ForceCollisionConstants) == 56
+    assert C.sizeof(gi.MoveCollisionConstants) == 16
+    assert C.sizeof(gi.ForceSeparateConstants) == 64
+    assert C.sizeof(gi.MoveSeparateConstants) == 24
+
+
+def test_fractions_must_add_to_one(gi):
+    s = gi.generator_setup_defaults(1024)
+    s.fraction_si_body1 = 0.7
+    u = gi.Units()
+    assert gi.load().gi_units_compute(C.byref(s), C.byref(u)) == -1
+
+
+def test_setup_file_is_read_positionally(gi):
+    """The template text of setupGeneratingTargetImpactor, key names ignored."""
+    s = gi.GeneratorSetup()
+    assert gi.load().gi_generator_setup_read(str(GOLDEN / "setupGeneratingTargetImpactor").encode(), C.byref(s)) == 0
+    d = gi.generator_setup_defaults(1024)
+    assert _bytes(s) == _bytes(d)
+    assert gi.load().gi_generator_setup_read(str(GOLDEN / "missing_file").encode(), C.byref(s)) == -5
+    trunc = GOLDEN / "setupGeneratingTargetImpactor.truncated"
+    assert gi.load().gi_generator_setup_read(str(trunc).encode(), C.byref(s)) == -6
+
+
+def test_parameter_file_round_trip(gi, tmp_path):
+    u = gi.units(gi.generator_setup_defaults(2048))
+    path = str(tmp_path / "targetImpactorParameters").encode()
+    assert gi.load().gi_parameters_write(path, C.byref(u), 2048) == 0
+    r, n = gi.Units(), C.c_int()
+    assert gi.load().gi_parameters_read(path, C.byref(r), C.byref(n)) == 0
+    assert n.value == 2048
+    assert r.mass_fe == 2.364458 and r.mass_si == 1.0 and r.diameter == 1.0
+    assert r.k_fe == round(u.k_fe, 6) and r.sd_si == 0.002
+    assert (r.nfe1, r.nsi1, r.nfe2, r.nsi2) == (u.nfe1, u.nsi1, u.nfe2, u.nsi2)
+    assert r.nfe == u.nfe and r.nsi == u.nsi
+    # the impact constants from the file equal those computed directly
+    fc1, _ = gi.impact_constants(u)
+    r.mass_fe, r.mass_si = r.mass_fe, r.mass_si
+    fc2, _ = gi.impact_constants(r)
+    assert _bytes(fc1) == _bytes(fc2)
+
+
+@pytest.mark.parametrize("nt,ni,spin", [(1536, 512, 0.0), (800, 200, 0.5), (4000, 0, 0.3)])
+def test_synthetic_impact_layout_and_kinematics(gi, nt, ni, spin):
+    pos, vel, fc, mc, info = gi.make_impact(nt, ni, seed=11, angle_deg=45, speed_vesc=1.2, target_spin=spin)
+    n = nt + ni
+    assert info.n == n and fc.NFe == info.nfe == info.nfe1 + info.nfe2 and mc.NFe == fc.NFe
+    assert info.nfe1 == int(np.floor(0.3 * nt + 0.5))
+    # layout [Fe1, Fe2, Si1, Si2] with the generator's tags in pos.w and masses in vel.w
+    tags = np.concatenate([np.full(info.nfe1, 0), np.full(info.nfe2, 2), np.full(info.nsi1, 1), np.full(info.nsi2, 3)])
+    assert np.array_equal(pos[:, 3], tags.astype(np.float32))
+    assert np.all(vel[: fc.NFe, 3] == np.float32(fc.GMassFeSi)) and np.all(vel[fc.NFe :, 3] == 1.0)
+    m = np.where(np.arange(n) < fc.NFe, float(fc.GMassFeSi), 1.0)
+    com = (m[:, None] * pos[:, :3]).sum(0) / m.sum()
+    mom = (m[:, None] * vel[:, :3]).sum(0)
+    assert np.abs(com).max() < 1e-4
+    assert np.abs(mom).max() < 1e-3 * m.sum()
+    # jittered lattice: no pair closer than 1 - 2*sqrt(3)*jitter
+    from scipy.spatial import cKDTree
+
+    t = cKDTree(pos[:, :3])
+    assert len(t.query_pairs(1 - 2 * np.sqrt(3) * 0.1 - 1e-6)) == 0
+    # iron is the core of the target: innermost by radius from the body centre
+    body = np.concatenate([np.arange(info.nfe1), np.arange(fc.NFe, fc.NFe + info.nsi1)])
+    c = pos[body, :3].mean(0)
+    r = np.linalg.norm(pos[body, :3] - c, axis=1)
+    assert r[: info.nfe1].max() < np.percentile(r[info.nfe1 :], 10) + 1.0
+    if spin > 0:
+        mb = m[body, None]
+        cm = (mb * pos[body, :3]).sum(0) / mb.sum()
+        vm = (mb * vel[body, :3]).sum(0) / mb.sum()
+        L = (mb * np.cross(pos[body, :3] - cm, vel[body, :3] - vm)).sum(0)
+        # rigid spin about +z: L = I omega, off-axis parts only from the small
+        # products of inertia of the jittered lattice
+        assert L[2] > 0 and abs(L[0]) < 1e-2 * L[2] and abs(L[1]) < 1e-2 * L[2]
+        r_perp2 = ((pos[body, :2] - cm[:2]) ** 2).sum(1)
+        assert L[2] == pytest.approx((m[body] * r_perp2).sum() * info.target_omega, rel=1e-3)
+
+
+def test_synthetic_impact_is_seeded(gi):
+    a = gi.make_impact(300, 100, seed=5)[0]
+    b = gi.make_impact(300, 100, seed=5)[0]
+    c = gi.make_impact(300, 100, seed=6)[0]
+    assert np.array_equal(a, b) and not np.array_equal(a, c)
+
+
+def test_invalid_arguments_are_rejected_before_touching_a_device(gi):
+    lib = gi.load()
+    fc, mc = gi.impact_constants(gi.units(gi.generator_setup_defaults(1024)))
+    vp = C.c_void_p
+    assert lib.gi_force_collision(vp(1), vp(1), vp(1), -1, C.byref(fc), None) == -1
+    fc.NFe = 99
+    assert lib.gi_force_collision(vp(1), vp(1), vp(1), 50, C.byref(fc), None) == -1
+    assert b"NFe" in lib.gi_last_error_string()
+    assert lib.gi_force_collision(None, None, None, 10, None, None) == -1
+    assert lib.gi_force_collision(None, None, None, 0, C.byref(fc), None) in (0, -1)
+    fs = gi.ForceSeparateConstants()
+    fs.boarder1, fs.boarder2, fs.boarder3 = 5, 3, 8
+    assert lib.gi_force_separate(vp(1), vp(1), vp(1), 10, C.byref(fs), None) == -1
+    assert lib.gi_move_collision(vp(1), vp(1), vp(1), -3, C.byref(mc), None) == -1
+    assert lib.gi_check_block_multiple(1000) == -2 and lib.gi_check_block_multiple(1024) == 0
+    assert lib.gi_set_tiling(3, 0, 0, 0) == -1 and lib.gi_set_tiling(0, 0, 0, 0) == 0
+    assert lib.gi_impact_steps(vp(1), vp(1), vp(1), 10, C.byref(fc), C.byref(mc), 5, 1, None, None, None) == -1
+
+
+def _declared_functions() -> list[str]:
+    names = []
+    for h in sorted((REPO / "include").glob("*.h")):
+        text = h.read_text()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names += re.findall(r"\b(gi_[a-z0-9_]+)\s*\(", text)
+    return sorted(set(names))
+
+
+def test_library_exports_every_declared_symbol(gi):
+    declared = _declared_functions()
+    assert len(declared) > 30
+    out = subprocess.run(["nm", "-D", "--defined-only", str(gi.LIB_PATH)], capture_output=True, text=True, check=True)
+    exported = set(re.findall(r"\bT (gi_[a-z0-9_]+)", out.stdout))
+    missing = [d for d in declared if d not in exported]
+    assert not missing, f"declared but not exported: {missing}"
+    lib = gi.load()
+    for d in declared:
+        assert getattr(lib, d) is not None

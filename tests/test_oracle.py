@@ -1,0 +1,366 @@
+"""Pins of the CPU oracle (oracle/gi_oracle.c) against what the physics and
+the reference's source fix, independently of the oracle's own formulas.
+
+The reference ships no tests, fixtures or golden vectors, so each check here
+is a closed form, an invariant, a special case or a known-answer value a
+plausible mistake would break (wrong sign, dropped mass factor, swapped
+material, wrong receding branch, kick/drift order, ...).
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+from conftest import f4
+
+
+def _consts(O, n=2048):
+    u = O.units(n)
+    fc, mc = O.impact_constants(u)
+    return u, fc, mc
+
+
+def _pair(p0, p1, v0=(0, 0, 0), v1=(0, 0, 0)):
+    pos, vel = f4(2), f4(2)
+    pos[0, :3], pos[1, :3] = p0, p1
+    vel[0, :3], vel[1, :3] = v0, v1
+    return pos, vel
+
+
+def _mag_on_0(O, fc, r, nfe, dvx=0.0):
+    """Force on particle 0 from particle 1 placed at +r along x (positive = attraction)."""
+    pos, vel = _pair((0, 0, 0), (r, 0, 0), (0, 0, 0), (dvx, 0, 0))
+    fc.NFe = nfe
+    F = O.force_collision_f32(pos, vel, fc)
+    assert F[0, 1] == 0 and F[0, 2] == 0
+    return float(F[0, 0]), F
+
+
+# --------------------------------------------------------------------------- gravity branch
+
+
+@pytest.mark.parametrize("nfe,expect_mass", [(0, "1"), (1, "FeSi"), (2, "FeFe")])
+def test_gravity_closed_form_and_third_law(oracle, nfe, expect_mass):
+    _, fc, _ = _consts(oracle)
+    mfe = 7.850e12 / 3.320e12  # MassFe = DensityFe / DensitySi
+    gm = {"1": 1.0, "FeSi": mfe, "FeFe": mfe * mfe}[expect_mass]
+    for r in (1.0, 2.0, 3.5, 10.0):
+        f0, F = _mag_on_0(oracle, fc, r, nfe)
+        assert f0 > 0, "gravity must pull particle 0 toward particle 1"
+        assert f0 == pytest.approx(gm / r**2, rel=2e-6)
+        assert F[1, 0] == -F[0, 0]  # Newton's third law, exactly
+
+
+def test_self_pair_contributes_nothing(oracle):
+    _, fc, _ = _consts(oracle)
+    pos, vel = f4(1), f4(1)
+    pos[0, :3] = (3.0, -2.0, 1.0)
+    fc.NFe = 1
+    assert np.all(oracle.force_collision_f32(pos, vel, fc)[:, :3] == 0)
+    F64, cond, _ = oracle.force_collision_f64(pos, vel, fc)
+    assert np.all(F64 == 0) and cond[0] == 0
+
+
+# --------------------------------------------------------------------------- contact branch
+
+
+def _potential(G, K, r):
+    """Pair potential of the Greenspan law: -G/r outside, spring-like inside."""
+    if r >= 1.0:
+        return -G / r
+    return G * (r - 2.0) + (K / 3.0) * (r - 1.0) ** 2 * (r + 2.0)
+
+
+@pytest.mark.parametrize("nfe", [0, 1, 2])
+def test_contact_force_is_derivative_of_potential(oracle, nfe):
+    """Approaching pairs (dv = 0): f(r) = U'(r) for the stated potential, in every
+    shell region, and f is continuous at r = 1."""
+    _, fc, _ = _consts(oracle)
+    G = [1.0, fc.GMassFeSi, fc.GMassFeFe][nfe]
+    K = [fc.KSiSi, fc.KFeSi, fc.KFeFe][nfe]
+    for r in (0.9999, 0.999, 0.9985, 0.99, 0.95, 0.9, 0.8, 0.6):
+        f, _ = _mag_on_0(oracle, fc, r, nfe)
+        h = 1e-6
+        du = (_potential(G, K, r + h) - _potential(G, K, r - h)) / (2 * h)
+        assert f == pytest.approx(du, rel=2e-5, abs=2e-4 * K * 1e-3)
+    r_in = float(np.nextafter(np.float32(1.0), np.float32(0.0)))
+    f_in, _ = _mag_on_0(oracle, fc, r_in, nfe)
+    f_out, _ = _mag_on_0(oracle, fc, 1.0, nfe)
+    assert f_out == pytest.approx(G, rel=1e-6)
+    # continuous at r = 1: the jump over one float step is the slope 2K r times the step
+    assert abs(f_in - f_out) <= 2.2 * K * (1.0 - r_in) + 1e-6 * G
+
+
+def test_fe_si_stiffness_is_the_mean_of_the_pure_pairs(oracle):
+    """KFeSi = KFe + KSi while KFeFe = 2 KFe and KSiSi = 2 KSi: the mixed overlap
+    force is the mean of the pure ones (after removing gravity)."""
+    _, fc, _ = _consts(oracle)
+    for r in (0.999, 0.99, 0.95):
+        ss, _ = _mag_on_0(oracle, fc, r, 0)
+        fs, _ = _mag_on_0(oracle, fc, r, 1)
+        ff, _ = _mag_on_0(oracle, fc, r, 2)
+        assert (fs - fc.GMassFeSi) == pytest.approx(0.5 * ((ss - 1.0) + (ff - fc.GMassFeFe)), rel=1e-5)
+
+
+@pytest.mark.parametrize(
+    "nfe,r,kr_field,k_field",
+    [
+        (0, 0.9, "KRSiSi", "KSiSi"),  # Si-Si below ShellBreakSi
+        (2, 0.95, "KRFeFe", "KFeFe"),  # Fe-Fe below ShellBreakFe
+        (1, 0.99, "KRMix", "KFeSi"),  # Fe-Si between the two shell breaks
+        (1, 0.95, "KRFeSi", "KFeSi"),  # Fe-Si below both
+    ],
+)
+def test_receding_uses_reduced_stiffness_only_when_int_dot_positive(oracle, nfe, r, kr_field, k_field):
+    """impactAddTime.cu:313-315: (int)(dp.dv) > 0 selects KR.  A positive dot
+    product below 1 truncates to 0 and keeps the approach stiffness."""
+    _, fc, _ = _consts(oracle)
+    G = [1.0, fc.GMassFeSi, fc.GMassFeFe][nfe]
+    approach, _ = _mag_on_0(oracle, fc, r, nfe, dvx=-1.0 / r)  # dot = -1
+    slow_apart, _ = _mag_on_0(oracle, fc, r, nfe, dvx=0.5 / r)  # dot = 0.5 -> (int) 0
+    apart, _ = _mag_on_0(oracle, fc, r, nfe, dvx=1.5 / r)  # dot = 1.5 -> (int) 1
+    assert slow_apart == approach
+    ratio = getattr(fc, kr_field) / getattr(fc, k_field)
+    assert (apart - G) == pytest.approx((approach - G) * ratio, rel=1e-5)
+    assert apart > approach  # less repulsion while receding
+
+
+def test_shell_region_ignores_receding(oracle):
+    """Between ShellBreak and 1 the approach stiffness applies both ways."""
+    _, fc, _ = _consts(oracle)
+    for nfe, r in ((0, 0.999), (2, 0.99), (1, 0.999)):
+        a, _ = _mag_on_0(oracle, fc, r, nfe, dvx=-3.0)
+        b, _ = _mag_on_0(oracle, fc, r, nfe, dvx=3.0)
+        assert a == b
+
+
+def test_trunc_to_int_matches_cvt_rzi(oracle):
+    assert oracle.trunc_to_int(0.999) == 0
+    assert oracle.trunc_to_int(1.0) == 1
+    assert oracle.trunc_to_int(-0.5) == 0
+    assert oracle.trunc_to_int(-1.5) == -1
+    assert oracle.trunc_to_int(float("nan")) == 0
+    assert oracle.trunc_to_int(1e10) == 2**31 - 1
+    assert oracle.trunc_to_int(-1e10) == -(2**31)
+
+
+# --------------------------------------------------------------------------- invariants
+
+
+def _cluster(n, seed, spacing=0.93, vscale=0.5):
+    rng = np.random.default_rng(seed)
+    k = int(math.ceil(n ** (1 / 3)))
+    g = np.stack(np.meshgrid(np.arange(k), np.arange(k), np.arange(k), indexing="ij"), -1).reshape(-1, 3)[:n]
+    pos, vel = f4(n), f4(n)
+    pos[:, :3] = g * spacing + rng.uniform(-0.05, 0.05, (n, 3))
+    vel[:, :3] = rng.normal(0, vscale, (n, 3))
+    return pos, vel
+
+
+def test_third_law_and_zero_torque_with_contacts(oracle):
+    _, fc, _ = _consts(oracle)
+    pos, vel = _cluster(300, 1)
+    fc.NFe = 90
+    F64, cond, _ = oracle.force_collision_f64(pos, vel, fc)
+    assert np.abs(F64.sum(0)).max() <= 1e-9 * cond.sum()
+    torque = np.cross(pos[:, :3].astype(np.float64), F64).sum(0)
+    assert np.abs(torque).max() <= 1e-9 * (cond * np.linalg.norm(pos[:, :3], axis=1)).sum()
+    F32 = oracle.force_collision_f32(pos, vel, fc)[:, :3].astype(np.float64)
+    assert np.abs(F32.sum(0)).max() <= 1e-5 * cond.sum()
+
+
+def test_float_restatement_agrees_with_fp64_twin(oracle):
+    _, fc, _ = _consts(oracle)
+    pos, vel = _cluster(512, 2, vscale=3.0)
+    fc.NFe = 150
+    F32 = oracle.force_collision_f32(pos, vel, fc)[:, :3].astype(np.float64)
+    F64, cond, near = oracle.force_collision_f64(pos, vel, fc)
+    err = np.linalg.norm(F32 - F64, axis=1) / cond
+    assert np.max(err[~near]) <= 1e-5
+
+
+def _run(oracle, pos, vel, fc, mc, steps):
+    p, v, _ = oracle.impact_steps_f32(pos, vel, fc, mc, steps)
+    return p, v
+
+
+def test_energy_conserved_with_kr_one_and_dissipated_with_kr_below_one(oracle):
+    """With KR = K the law is conservative (the potential above); with the
+    template KR = 0.01 K rebounds lose energy."""
+    _, fc, mc = _consts(oracle)
+    pos, vel = _cluster(27, 3, spacing=0.95, vscale=2.0)
+    fc.NFe = mc.NFe = 8
+    mc.Dt = 2e-5
+    mc.DtOverMassFe = np.float32(2e-5 / fc.GMassFeSi)
+    mc.DtOverMassSi = np.float32(2e-5)
+    cons = oracle.FC()
+    for k, _ in oracle.FC._fields_:
+        setattr(cons, k, getattr(fc, k))
+    cons.KRFeFe, cons.KRSiSi, cons.KRFeSi, cons.KRMix = cons.KFeFe, cons.KSiSi, cons.KFeSi, cons.KFeSi
+    E0 = oracle.conserved_f64(pos, vel, cons)
+    scale = abs(E0[0]) + abs(E0[1] - oracle.conserved_f64(pos * 50, vel, cons)[1])
+    p, v = _run(oracle, pos, vel, cons, mc, 3000)
+    E1 = oracle.conserved_f64(p, v, cons)
+    assert abs(E1[2] - E0[2]) <= 2e-3 * scale
+    # same start, reduced rebound stiffness: energy goes down
+    p2, v2 = _run(oracle, pos, vel, fc, mc, 3000)
+    E2 = oracle.conserved_f64(p2, v2, fc)
+    assert E2[2] < E0[2] - 1e-2 * scale
+
+
+def test_momentum_and_angular_momentum_conserved(oracle):
+    _, fc, mc = _consts(oracle)
+    pos, vel = _cluster(200, 4, spacing=0.97, vscale=1.0)
+    c = pos[:, :3].mean(0)
+    vel[:, 0] += -(pos[:, 1] - c[1]) * 0.5
+    vel[:, 1] += (pos[:, 0] - c[0]) * 0.5
+    fc.NFe = mc.NFe = 60
+    E0 = oracle.conserved_f64(pos, vel, fc)
+    p, v = _run(oracle, pos, vel, fc, mc, 200)
+    E1 = oracle.conserved_f64(p, v, fc)
+    mom_scale = np.abs(vel[:, :3]).sum() * 2.4
+    assert np.abs(E1[3:6] - E0[3:6]).max() <= 1e-4 * mom_scale
+    assert np.abs(E1[6:9] - E0[6:9]).max() <= 1e-3 * np.abs(E0[6:9]).max()
+
+
+def test_kepler_circular_orbit(oracle):
+    """Two silicate elements 10 diameters apart on a circular orbit keep their
+    separation and complete one revolution in 2 pi sqrt(d^3 / (G M))."""
+    _, fc, mc = _consts(oracle)
+    d = 10.0
+    v = math.sqrt(2.0 / d) / 2.0  # each body moves at half the relative speed
+    pos, vel = _pair((-d / 2, 0, 0), (d / 2, 0, 0), (0, -v, 0), (0, v, 0))
+    fc.NFe = mc.NFe = 0
+    period = 2 * math.pi * math.sqrt(d**3 / 2.0)
+    steps = int(round(period / mc.Dt))
+    p, _ = _run(oracle, pos, vel, fc, mc, steps)
+    sep = p[1, :3] - p[0, :3]
+    assert np.linalg.norm(sep) == pytest.approx(d, rel=2e-3)
+    ang = math.atan2(sep[1], sep[0])
+    assert abs(ang) < 2e-2
+
+
+# --------------------------------------------------------------------------- move kernels
+
+
+def test_move_is_kick_then_drift_with_new_velocity(oracle):
+    _, fc, mc = _consts(oracle)
+    pos, vel, frc = f4(3), f4(3), f4(3)
+    pos[:, :3] = [[1, 2, 3], [-1, 0.5, 2], [0, 0, 0]]
+    vel[:, :3] = [[0.1, -0.2, 0.3], [1, 1, 1], [0, 0, 0]]
+    frc[:, :3] = [[4, 5, 6], [-7, 8, -9], [1, 1, 1]]
+    pos[:, 3], vel[:, 3], frc[:, 3] = 7.0, 8.0, 9.0
+    mc.NFe = 1
+    p, v = oracle.move_collision_f32(pos, vel, frc, mc)
+    t = np.array([mc.DtOverMassFe, mc.DtOverMassSi, mc.DtOverMassSi], np.float32)[:, None]
+    v_exp = (vel[:, :3] + frc[:, :3] * t).astype(np.float32)
+    p_exp = (pos[:, :3] + v_exp * np.float32(mc.Dt)).astype(np.float32)
+    assert np.array_equal(v[:, :3], v_exp) and np.array_equal(p[:, :3], p_exp)
+    assert not np.array_equal(p[:2, :3], (pos[:2, :3] + vel[:2, :3] * np.float32(mc.Dt)))
+    assert np.all(p[:, 3] == 7.0) and np.all(v[:, 3] == 8.0)
+
+
+def test_damped_move_uses_body_damping_and_material_mass(oracle):
+    u = oracle.units(2048)
+    fs, ms = oracle.separate_constants(u)
+    ms.boarder1, ms.boarder2, ms.boarder3 = 1, 2, 3  # one particle per [Fe1, Si1, Fe2, Si2]
+    pos, vel, frc = f4(4), f4(4), f4(4)
+    vel[:, :3] = 2.0
+    frc[:, :3] = 3.0
+    d1, d2 = 8.0, 0.5
+    p, v = oracle.move_damped_separate_f32(pos, vel, frc, ms, d1, d2)
+    t = [ms.DtOverMassFe, ms.DtOverMassSi, ms.DtOverMassFe, ms.DtOverMassSi]
+    d = [d1, d1, d2, d2]
+    for i in range(4):
+        exp = np.float32(2.0) + (np.float32(3.0) - np.float32(d[i]) * np.float32(2.0)) * np.float32(t[i])
+        assert v[i, 0] == np.float32(exp)
+
+
+def test_separate_force_equals_each_body_alone(oracle):
+    """Cross-body pairs are skipped (generateTargetImpactor.cu:529): the force on
+    body 1 is that of body 1 alone, even with both bodies overlapping in space."""
+    u = oracle.units(2048)
+    fs, _ = oracle.separate_constants(u)
+    pos1, vel1 = _cluster(150, 5, vscale=2.0)
+    pos2, vel2 = _cluster(110, 6, vscale=2.0)
+    pos2[:, :3] += 0.4  # interpenetrating
+    b1, b2, b3 = 40, 150, 150 + 30
+    fs.boarder1, fs.boarder2, fs.boarder3 = b1, b2, b3
+    pos = np.concatenate([pos1, pos2])
+    vel = np.concatenate([vel1, vel2])
+    F = oracle.force_separate_f32(pos, vel, fs)
+    fc = oracle.FC()
+    for k, _ in oracle.FC._fields_[:-1]:
+        setattr(fc, k, getattr(fs, k))
+    fc.NFe = b1
+    assert np.array_equal(F[:b2, :3], oracle.force_collision_f32(pos1, vel1, fc)[:, :3])
+    fc.NFe = b3 - b2
+    assert np.array_equal(F[b2:, :3], oracle.force_collision_f32(pos2, vel2, fc)[:, :3])
+
+
+# --------------------------------------------------------------------------- units and host helpers
+
+
+@pytest.mark.parametrize(
+    "n,nfe,nsi,length_km,time_s,k_sim",
+    [
+        (2048, 264, 1784, 1161.65, 2935.86, 551.16),
+        (65536, 8478, 57058, 365.96, 2935.86, 5553.37),
+        (262144, 33914, 228230, 230.54, 2935.86, 13993.53),
+    ],
+)
+def test_units_known_answers(oracle, n, nfe, nsi, length_km, time_s, k_sim):
+    u = oracle.units(n)
+    assert (u.nfe, u.nsi) == (nfe, nsi)
+    assert u.unit_length == pytest.approx(length_km, abs=0.01)
+    assert u.unit_time == pytest.approx(time_s, abs=0.01)
+    assert u.k_fe_sim == pytest.approx(k_sim, abs=0.01)
+    # closed forms: G = 1 in these units, element mass ratio = density ratio
+    assert u.unit_time == pytest.approx(math.sqrt(6.0 / (3.141592654 * 6.67408e-20 * 3.320e12)), rel=1e-12)
+    assert u.mass_fe == pytest.approx(7.850 / 3.320, rel=1e-15)
+    vol = 3.141592654 / 6 * u.unit_length**3  # the reference's Pi (commonSetup.h:29)
+    assert vol * 3.320e12 == pytest.approx(u.unit_mass, rel=1e-12)
+    assert u.nfe1 + u.nfe2 == u.nfe and u.nsi1 + u.nsi2 == u.nsi
+
+
+def test_percent_f_round_trip_of_impact_constants(oracle):
+    assert oracle.load().or_round_f(2.3644578313253013) == 2.364458
+    u = oracle.units(2048)
+    fc, mc = oracle.impact_constants(u)
+    assert fc.GMassFeSi == np.float32(2.364458)
+    assert fc.GMassFeFe == np.float32(2.364458 * 2.364458)
+    assert fc.ShellBreakFe == np.float32(0.98) and fc.ShellBreakSi == np.float32(0.998)
+    assert fc.ShellBreakFeSi1 == np.float32(0.998) and fc.ShellBreakFeSi2 == np.float32(0.98)
+    assert fc.KRMix == np.float32(round(u.k_fe_sim, 6) + round(u.k_si_sim, 6) * 0.01)
+    assert mc.DtOverMassFe == np.float32(0.002 / 2.364458)
+
+
+def test_recenter_restores_target_com_and_velocity(oracle):
+    pos, vel = _cluster(500, 7, vscale=1.0)
+    pos[:, :3] += 30.0
+    mfe = 2.364458
+    p, v = oracle.recenter_f32(pos, vel, 100, mfe, 1.0, (1.0, 2.0, 3.0), (0.1, 0.2, 0.3))
+    cv = oracle.com_vel_collision_f32(p, v, 100, mfe, 1.0)
+    assert np.allclose(cv[:3], [1, 2, 3], atol=1e-4)
+    assert np.allclose(cv[3:], [0.1, 0.2, 0.3], atol=1e-5)
+    m = np.where(np.arange(500) < 100, mfe, 1.0)
+    com64 = (m[:, None] * p[:, :3].astype(np.float64)).sum(0) / m.sum()
+    assert np.allclose(com64, [1, 2, 3], atol=1e-3)
+
+
+def test_spin_body_is_rigid_rotation(oracle):
+    """spinBody adds v = 2 pi w (n x r) about the body's centre of mass."""
+    pos, vel = _cluster(343, 8, spacing=1.0)
+    vel[:] = 0
+    nfe, mfe = 100, 2.364458
+    total = nfe * mfe + (343 - nfe)
+    w = 0.05
+    v = oracle.spin_body(pos, vel, 0, 343, nfe, mfe, 1.0, total, (0.0, 0.0, 2.0, w))
+    com = oracle.body_com(pos, 0, nfe, 343, mfe, 1.0, total)[:3]
+    r = pos[:, :3].astype(np.float64) - com
+    expect = 2 * math.pi * w * np.stack([-r[:, 1], r[:, 0], np.zeros(343)], 1)
+    assert np.allclose(v[:, :3], expect, rtol=1e-5, atol=1e-6)
+    with pytest.raises(ValueError):
+        oracle.spin_body(pos, vel, 0, 343, nfe, mfe, 1.0, total, (0.0, 0.0, 0.0, w))
