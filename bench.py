@@ -56,7 +56,8 @@ def cpu_baseline(pos, vel, fc, n, target_s: float):
     from oracle import oracle as O
 
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
-    rows = 64
+    rows = 64 * threads  # every thread busy during the calibration
+    O.force_collision_f32(pos, vel, fc, 0, 16 * threads, nthreads=threads)  # thread pool start-up
     t0 = time.perf_counter()
     O.force_collision_f32(pos, vel, fc, 0, rows, nthreads=threads)
     dt = time.perf_counter() - t0

@@ -87,7 +87,7 @@ struct Plan {
     std::vector<SimDesc> sims;
     std::vector<GravTask> tasks;
     std::vector<int> block_sim, block_first, sim_first_block;
-    Tiling tiling{4, 0, 0, 0};
+    Tiling tiling{4, 0, 0, 0, 0};
     int ntot = 0;
     int partial_len = 0;  // float4 elements
     unsigned mask = 0;    // hash table size - 1
@@ -173,8 +173,8 @@ static bool same_sims(const std::vector<SimDesc>& a, const std::vector<SimDesc>&
 }
 
 static bool same_tiling(const Tiling& a, const Tiling& b) {
-    return a.R == b.R && a.chunk_len == b.chunk_len && a.target_tasks == b.target_tasks &&
-           a.min_chunk == b.min_chunk;
+    return a.R == b.R && a.variant == b.variant && a.chunk_len == b.chunk_len &&
+           a.target_tasks == b.target_tasks && a.min_chunk == b.min_chunk;
 }
 
 // Fill nchunk / partial_base of the simulations and build the task table.
@@ -350,17 +350,20 @@ static TimingPair take_pair(DevCtx* ctx, int kind) {
     return t;
 }
 
-template <bool POT>
+template <bool POT, int V>
 static void launch_gravity_kernel(int R, dim3 grid, dim3 block, hipStream_t st, const float4* pos,
                                   const Workspace* ws, int ntasks) {
     if (R == 8)
-        hipLaunchKernelGGL((k_gravity<8, 128, POT>), grid, block, 0, st, pos, ws->tasks.p, ntasks, ws->sims.p,
+        hipLaunchKernelGGL((k_gravity<8, 128, POT, V>), grid, block, 0, st, pos, ws->tasks.p, ntasks, ws->sims.p,
                            ws->partial.p);
     else if (R == 2)
-        hipLaunchKernelGGL((k_gravity<2, 128, POT>), grid, block, 0, st, pos, ws->tasks.p, ntasks, ws->sims.p,
+        hipLaunchKernelGGL((k_gravity<2, 128, POT, V>), grid, block, 0, st, pos, ws->tasks.p, ntasks, ws->sims.p,
+                           ws->partial.p);
+    else if (R == 1)
+        hipLaunchKernelGGL((k_gravity<1, 128, POT, V>), grid, block, 0, st, pos, ws->tasks.p, ntasks, ws->sims.p,
                            ws->partial.p);
     else
-        hipLaunchKernelGGL((k_gravity<4, 128, POT>), grid, block, 0, st, pos, ws->tasks.p, ntasks, ws->sims.p,
+        hipLaunchKernelGGL((k_gravity<4, 128, POT, V>), grid, block, 0, st, pos, ws->tasks.p, ntasks, ws->sims.p,
                            ws->partial.p);
 }
 
@@ -377,9 +380,11 @@ static int launch_gravity(DevCtx* ctx, Workspace* ws, const float4* pos, bool po
         GI_HIP(hipEventRecord(tp.a, st));
     }
     if (pot)
-        launch_gravity_kernel<true>(p.tiling.R, grid, block, st, pos, ws, ntasks);
+        launch_gravity_kernel<true, 0>(p.tiling.R, grid, block, st, pos, ws, ntasks);
+    else if (p.tiling.variant == 1)
+        launch_gravity_kernel<false, 1>(p.tiling.R, grid, block, st, pos, ws, ntasks);
     else
-        launch_gravity_kernel<false>(p.tiling.R, grid, block, st, pos, ws, ntasks);
+        launch_gravity_kernel<false, 0>(p.tiling.R, grid, block, st, pos, ws, ntasks);
     GI_HIP(hipGetLastError());
     if (timed) {
         GI_HIP(hipEventRecord(tp.b, st));
@@ -397,7 +402,7 @@ static int run_force(DevCtx* ctx, Workspace* ws, const float4* pos, const float4
     if (rc) return rc;
     if ((rc = launch_gravity(ctx, ws, pos, false, st, timed))) return rc;
     hipLaunchKernelGGL(k_finish, dim3(nblk(p.ntot, 256)), dim3(256), 0, st, pos, vel, force, p.ntot, ws->sims.p,
-                       (int)p.sims.size(), ws->partial.p, ws->starts.p, ws->sorted.p, p.mask);
+                       (int)p.sims.size(), ws->partial.p, ws->starts.p, ws->sorted.p, p.mask, p.tiling.variant);
     GI_HIP(hipGetLastError());
     return 0;
 }
@@ -644,16 +649,18 @@ int gi_mem_info(size_t* free_bytes, size_t* total_bytes) {
     return 0;
 }
 
-int gi_set_tiling(int r, long long chunk_len, long long target_tasks, long long min_chunk) {
+int gi_set_tiling(int r, int variant, long long chunk_len, long long target_tasks, long long min_chunk) {
     if (r == 0) {
         tiling_override().active = 0;
         return 0;
     }
-    if (r != 2 && r != 4 && r != 8) return set_error(GI_E_INVALID_ARG, "gi_set_tiling: r must be 2, 4 or 8");
+    if (r != 1 && r != 2 && r != 4 && r != 8)
+        return set_error(GI_E_INVALID_ARG, "gi_set_tiling: r must be 1, 2, 4 or 8");
+    if (variant != 0 && variant != 1) return set_error(GI_E_INVALID_ARG, "gi_set_tiling: variant must be 0 or 1");
     if (chunk_len < 0 || target_tasks < 0 || min_chunk < 0)
         return set_error(GI_E_INVALID_ARG, "gi_set_tiling: negative size");
     tiling_override().active = 1;
-    tiling_override().t = Tiling{r, chunk_len, target_tasks > 0 ? target_tasks : 8192, min_chunk};
+    tiling_override().t = Tiling{r, variant, chunk_len, target_tasks > 0 ? target_tasks : 8192, min_chunk};
     return 0;
 }
 

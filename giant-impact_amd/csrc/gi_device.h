@@ -99,4 +99,22 @@ __device__ __forceinline__ float fast_inv_r3(float r2) {
     return ri * ri * ri;
 }
 
+// Variant 1: r^-3 = rsq(r^2) * rcp(r^2): one multiply and two transcendental
+// ops, which issue beside the ordinary VALU stream.
+__device__ __forceinline__ float fast_inv_r3_rcp(float r2) {
+    return __builtin_amdgcn_rsqf(r2) * __builtin_amdgcn_rcpf(r2);
+}
+
+template <int V>
+__device__ __forceinline__ float inv_r3(float r2) {
+    if constexpr (V == 1)
+        return fast_inv_r3_rcp(r2);
+    else
+        return fast_inv_r3(r2);
+}
+
+__device__ __forceinline__ float inv_r3_variant(int v, float r2) {
+    return v == 1 ? fast_inv_r3_rcp(r2) : fast_inv_r3(r2);
+}
+
 }  // namespace gi

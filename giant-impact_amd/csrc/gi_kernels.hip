@@ -37,7 +37,8 @@ __device__ __forceinline__ void kahan_add(float& sum, float& comp, float v) {
 
 // R: i-particles per lane.  T: j per float tile between Kahan folds.
 // POT: accumulate sum_j 1/r (self excluded) instead of sum_j dp/r^3.
-template <int R, int T, bool POT>
+// V: how r^-3 is formed (0: rsq cubed, 1: rsq times rcp).
+template <int R, int T, bool POT, int V>
 __global__ __launch_bounds__(256) void k_gravity(const float4* __restrict__ pos,
                                                  const GravTask* __restrict__ tasks, int ntasks,
                                                  const SimDesc* __restrict__ sims,
@@ -81,7 +82,7 @@ __global__ __launch_bounds__(256) void k_gravity(const float4* __restrict__ pos,
                 float dy = pj.y - yi[r];
                 float dz = pj.z - zi[r];
                 if constexpr (!POT) {
-                    float s = fast_inv_r3(fast_r2(dx, dy, dz));
+                    float s = inv_r3<V>(fast_r2(dx, dy, dz));
                     tx[r] = __builtin_fmaf(s, dx, tx[r]);
                     ty[r] = __builtin_fmaf(s, dy, ty[r]);
                     tz[r] = __builtin_fmaf(s, dz, tz[r]);
@@ -255,7 +256,8 @@ __global__ __launch_bounds__(256) void k_finish(const float4* __restrict__ pos,
                                                 const SimDesc* __restrict__ sims, int nsims,
                                                 const float4* __restrict__ partial,
                                                 const int* __restrict__ starts,
-                                                const int* __restrict__ sorted, unsigned mask) {
+                                                const int* __restrict__ sorted, unsigned mask,
+                                                int variant) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ntot) return;
     const int s = find_sim(sims, nsims, i);
@@ -308,7 +310,7 @@ __global__ __launch_bounds__(256) void k_finish(const float4* __restrict__ pos,
                     const float fm = contact_pair_mag(sd.law, test, r, r2, dot);
                     // What pass 1 added for this pair.
                     const float g = gravity_product(sd.law, test);
-                    const float sf = g * fast_inv_r3(fast_r2(dpx, dpy, dpz));
+                    const float sf = g * inv_r3_variant(variant, fast_r2(dpx, dpy, dpz));
                     ex += (double)(fm * dpx * invr) - (double)(sf * dpx);
                     ey += (double)(fm * dpy * invr) - (double)(sf * dpy);
                     ez += (double)(fm * dpz * invr) - (double)(sf * dpz);
@@ -600,11 +602,20 @@ __global__ void k_recenter_apply(float4* __restrict__ pos, float4* __restrict__ 
 }
 
 // Explicit instantiations used by the host code.
-template __global__ void k_gravity<4, 128, false>(const float4*, const GravTask*, int, const SimDesc*, float4*);
-template __global__ void k_gravity<8, 128, false>(const float4*, const GravTask*, int, const SimDesc*, float4*);
-template __global__ void k_gravity<2, 128, false>(const float4*, const GravTask*, int, const SimDesc*, float4*);
-template __global__ void k_gravity<2, 128, true>(const float4*, const GravTask*, int, const SimDesc*, float4*);
-template __global__ void k_gravity<4, 128, true>(const float4*, const GravTask*, int, const SimDesc*, float4*);
-template __global__ void k_gravity<8, 128, true>(const float4*, const GravTask*, int, const SimDesc*, float4*);
+#define GI_GRAVITY_INST(R, POT, V) \
+    template __global__ void k_gravity<R, 128, POT, V>(const float4*, const GravTask*, int, const SimDesc*, float4*);
+GI_GRAVITY_INST(1, false, 0)
+GI_GRAVITY_INST(1, false, 1)
+GI_GRAVITY_INST(1, true, 0)
+GI_GRAVITY_INST(2, false, 0)
+GI_GRAVITY_INST(4, false, 0)
+GI_GRAVITY_INST(8, false, 0)
+GI_GRAVITY_INST(2, false, 1)
+GI_GRAVITY_INST(4, false, 1)
+GI_GRAVITY_INST(8, false, 1)
+GI_GRAVITY_INST(2, true, 0)
+GI_GRAVITY_INST(4, true, 0)
+GI_GRAVITY_INST(8, true, 0)
+#undef GI_GRAVITY_INST
 
 }  // namespace gi

@@ -1,26 +1,42 @@
-// gi_tuning.h -- tile sizes of the all-pairs pass.
+// gi_tuning.h -- tiling of the all-pairs pass and its tuning table.
 //
-// R           i-particles held in registers per lane (a wavefront covers 64*R)
-// chunk_len   j-particles per task (0 = derive from target_tasks)
-// target_tasks number of wavefront tasks to aim for (several per SIMD of the
-//             256 CUs x 4 SIMDs so the tail of the launch stays short)
-// min_chunk   lower bound on chunk_len (each chunk costs one partial slot
-//             per particle)
+// R            i-particles held in registers per lane (a wavefront covers 64*R)
+// variant      how r^-3 is formed: 0 = rsq cubed (3 multiplies' worth of
+//              VALU), 1 = rsq * rcp (1 multiply, 2 transcendental ops)
+// chunk_len    j-particles per task (0 = derive from target_tasks)
+// target_tasks wavefront tasks to aim for (several per SIMD of the 256 CUs x
+//              4 SIMDs so the tail of the launch stays short)
+// min_chunk    lower bound on chunk_len (each chunk costs one partial slot
+//              per particle)
+//
+// The table is written by the autotuner (gi_amd.autotune) to
+// tuning/<arch>.txt next to the library's lib/ directory, one row per size
+// class: "n_max R variant chunk_len target_tasks min_chunk" (particles of
+// the whole launch, rows in increasing n_max).  Without a table the defaults
+// below apply.
 #pragma once
+
+#include <dlfcn.h>
+
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
 
 namespace gi {
 
 struct Tiling {
     int R;
+    int variant;
     long long chunk_len;
     long long target_tasks;
     long long min_chunk;
 };
 
-// Overridable at run time by gi_set_tiling (the autotuner's entry point).
 struct TilingOverride {
     int active = 0;
-    Tiling t{4, 0, 8192, 512};
+    Tiling t{4, 0, 0, 8192, 512};
 };
 
 inline TilingOverride& tiling_override() {
@@ -28,11 +44,50 @@ inline TilingOverride& tiling_override() {
     return o;
 }
 
+struct TuningRow {
+    long long n_max;
+    Tiling t;
+};
+
+inline void tuning_anchor() {}
+
+// Rows of tuning/gfx950.txt, located relative to this shared library.
+inline const std::vector<TuningRow>& tuning_table() {
+    static std::vector<TuningRow> rows;
+    static std::once_flag once;
+    std::call_once(once, [] {
+        Dl_info info;
+        if (!dladdr((void*)&tuning_anchor, &info) || !info.dli_fname) return;
+        std::string path(info.dli_fname);
+        size_t cut = path.rfind('/');
+        if (cut == std::string::npos) return;
+        path = path.substr(0, cut);  // .../lib
+        cut = path.rfind('/');
+        path = (cut == std::string::npos ? std::string(".") : path.substr(0, cut)) + "/tuning/gfx950.txt";
+        FILE* f = std::fopen(path.c_str(), "r");
+        if (!f) return;
+        char line[256];
+        while (std::fgets(line, sizeof line, f)) {
+            if (line[0] == '#') continue;
+            TuningRow r;
+            if (std::sscanf(line, "%lld %d %d %lld %lld %lld", &r.n_max, &r.t.R, &r.t.variant, &r.t.chunk_len,
+                            &r.t.target_tasks, &r.t.min_chunk) == 6 &&
+                (r.t.R == 1 || r.t.R == 2 || r.t.R == 4 || r.t.R == 8) && (r.t.variant == 0 || r.t.variant == 1) &&
+                r.t.target_tasks > 0)
+                rows.push_back(r);
+        }
+        std::fclose(f);
+    });
+    return rows;
+}
+
 inline Tiling choose_tiling(long long ntot, long long pairs, int nsims) {
-    if (tiling_override().active) return tiling_override().t;
     (void)pairs;
     (void)nsims;
-    Tiling t{4, 0, 8192, 512};
+    if (tiling_override().active) return tiling_override().t;
+    for (const TuningRow& r : tuning_table())
+        if (ntot <= r.n_max) return r.t;
+    Tiling t{4, 0, 0, 8192, 512};
     if (ntot < 4096) t.min_chunk = 128;
     return t;
 }

@@ -260,7 +260,7 @@ def load() -> C.CDLL:
         "gi_ensemble_set_active": (i, [vp, P(i)]),
         "gi_ensemble_conserved": (i, [vp, P(C.c_double)]),
         "gi_ensemble_work": (i, [vp, P(C.c_longlong), P(C.c_double)]),
-        "gi_set_tiling": (i, [i, C.c_longlong, C.c_longlong, C.c_longlong]),
+        "gi_set_tiling": (i, [i, i, C.c_longlong, C.c_longlong, C.c_longlong]),
     }
     for name, (res, args) in _optional.items():
         if hasattr(lib, name):

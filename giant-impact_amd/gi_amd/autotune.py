@@ -1,0 +1,102 @@
+"""Autotuner of the all-pairs pass (run on the GPU).
+
+For each size class it times the all-pairs gravity kernel (HIP events, via
+gi_kernel_timing) over a grid of tilings -- R (i-particles per lane), the
+r^-3 variant and the number of wavefront tasks -- and writes the fastest to
+tuning/gfx950.txt, which libgiant_impact.so reads on first use:
+
+    n_max R variant chunk_len target_tasks min_chunk
+
+Usage (on the GPU box):  python -m gi_amd.autotune [--quick] [--out PATH]
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import itertools
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+from . import DeviceArray, call, device_count, force_collision, load, make_impact, sync
+
+TUNING_DIR = Path(__file__).resolve().parent.parent / "tuning"
+
+# (n_max of the class, n_target, n_impactor of the representative impact)
+CLASSES = [
+    (6144, 3072, 1024),
+    (24576, 12288, 4096),
+    (98304, 49152, 16384),
+    (393216, 196608, 65536),
+    (2097152, 786432, 262144),
+]
+
+
+def time_gravity(n_target: int, n_impactor: int, reps: int):
+    pos, vel, fc, _, _ = make_impact(n_target, n_impactor, seed=77)
+    n = len(pos)
+    P, V, F = DeviceArray(n, pos), DeviceArray(n, vel), DeviceArray(n, np.zeros((n, 4), np.float32))
+    lib = load()
+
+    def run() -> float:
+        force_collision(P, V, F, n, fc)  # plan + warm
+        sync()
+        lib.gi_set_kernel_timing(1)
+        lib.gi_reset_kernel_timing()
+        for _ in range(reps):
+            force_collision(P, V, F, n, fc)
+        sync()
+        lib.gi_set_kernel_timing(0)
+        L, T = C.c_longlong(), C.c_double()
+        call("gi_kernel_timing", b"gravity", C.byref(L), C.byref(T))
+        return T.value / max(L.value, 1)
+
+    return run, n
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--quick", action="store_true", help="fewer candidates (smoke run)")
+    ap.add_argument("--out", default=str(TUNING_DIR / "gfx950.txt"))
+    ap.add_argument("--max-class", type=int, default=len(CLASSES))
+    ap.add_argument("--variants", action="store_true", help="also sweep the rsq*rcp variant")
+    args = ap.parse_args(argv)
+    if device_count() < 1:
+        print("autotune: no HIP device", file=sys.stderr)
+        return 1
+    lib = load()
+    Rs = [1, 2, 4, 8]
+    # variant 1 (rsq * rcp) lost to variant 0 at every size on gfx950 (20% slower
+    # at N = 262,144); --variants sweeps it again
+    variants = [0, 1] if args.variants else [0]
+    targets = [4096, 8192, 16384, 32768, 65536, 131072] if not args.quick else [32768]
+    rows = []
+    for n_max, nt, ni in CLASSES[: args.max_class]:
+        run, n = time_gravity(nt, ni, reps=3 if nt + ni > 100000 else 10)
+        best = None
+        for R, v, tt in itertools.product(Rs, variants, targets):
+            call("gi_set_tiling", R, v, 0, tt, 128 if n < 4096 else 512)
+            ms = run()
+            pairs = n * n / (ms / 1e3)
+            print(f"n={n:8d} R={R} v={v} tasks={tt:6d}: {ms:9.4f} ms  {pairs:.3e} pair/s", flush=True)
+            if best is None or ms < best[0]:
+                best = (ms, R, v, tt)
+        ms, R, v, tt = best
+        print(f"# class n<={n_max}: best R={R} variant={v} tasks={tt} ({n * n / (ms / 1e3):.3e} pair/s)", flush=True)
+        rows.append((n_max, R, v, 0, tt, 128 if n < 4096 else 512))
+    call("gi_set_tiling", 0, 0, 0, 0, 0)
+    out = Path(args.out)
+    out.parent.mkdir(parents=True, exist_ok=True)
+    with open(out, "w") as f:
+        f.write("# all-pairs tiling for gfx950, written by gi_amd.autotune on " + time.strftime("%Y-%m-%d") + "\n")
+        f.write("# n_max R variant chunk_len target_tasks min_chunk\n")
+        for r in rows:
+            f.write(" ".join(str(x) for x in r) + "\n")
+    print(f"wrote {out}")
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

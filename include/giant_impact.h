@@ -238,12 +238,14 @@ int gi_ensemble_work(gi_ensemble *e, long long *particles, double *pairs);
 
 /* ---- tiling of the all-pairs pass (the autotuner's control) -------------- */
 
-/* Override the tiling chosen from the built-in tuning table: r = i-particles
- * per lane (2, 4 or 8), chunk_len = j-particles per task (0 = derived from
- * target_tasks), target_tasks = wavefront tasks to aim for, min_chunk = lower
- * bound of chunk_len.  r = 0 restores the built-in table.  Plans built with
- * another tiling are rebuilt on their next use. */
-int gi_set_tiling(int r, long long chunk_len, long long target_tasks, long long min_chunk);
+/* Override the tiling chosen from the tuning table (tuning/gfx950.txt beside
+ * the library's lib/ directory, written by the autotuner): r = i-particles per
+ * lane (1, 2, 4 or 8), variant = how r^-3 is formed (0: rsq cubed, 1: rsq*rcp),
+ * chunk_len = j-particles per task (0 = derived from target_tasks),
+ * target_tasks = wavefront tasks to aim for, min_chunk = lower bound of
+ * chunk_len.  r = 0 restores the table.  Plans built with another tiling are
+ * rebuilt on their next use. */
+int gi_set_tiling(int r, int variant, long long chunk_len, long long target_tasks, long long min_chunk);
 
 /* ---- kernel timing (HIP events around the all-pairs force kernel) ------- */
 
