@@ -76,6 +76,60 @@ def cpu_baseline(pos, vel, fc, n, target_s: float):
     }
 
 
+def ensemble_leg(gi, n_sims: int, steps: int) -> dict:
+    """64 independent impacts of uneven N (U[4096, 32768], impactor fraction
+    [0.1, 0.3], angle [30, 60] deg, speed [1.0, 1.4] v_esc, target spin [0, 0.6]
+    breakup), batched vs each simulation stepped alone with the plain path
+    (gi_impact_steps, also graph-captured; its per-simulation set-up is
+    excluded from the sequential time)."""
+    cfg = gi.ensemble_configs(n_sims)
+    pos, vel, n, fc, mc, _ = gi.make_ensemble(cfg)
+    ntot = int(n.sum())
+    pairs = float((n.astype(np.float64) ** 2).sum())
+    P, V = gi.DeviceArray(ntot, pos), gi.DeviceArray(ntot, vel)
+    ens = gi.Ensemble(P, V, n, fc, mc)
+    c0 = ens.conserved()
+    ens.steps(1)
+    gi.sync()
+    t0 = time.perf_counter()
+    ens.steps(steps)
+    gi.sync()
+    t_batched = time.perf_counter() - t0
+    c1 = ens.conserved()
+    ens.close()
+    # sequential: every simulation alone, one after another
+    P.upload(pos)
+    V.upload(vel)
+    F = gi.DeviceArray(ntot, np.zeros((ntot, 4), np.float32))
+    off = np.concatenate([[0], np.cumsum(n)])
+    t_seq = 0.0
+    for s in range(n_sims):
+        ptr = lambda A: A.value + 16 * int(off[s])
+        gi.impact_steps(ptr(P), ptr(V), ptr(F), int(n[s]), fc[s], mc[s], 1)  # plan + capture
+        gi.sync()
+        t0 = time.perf_counter()
+        gi.impact_steps(ptr(P), ptr(V), ptr(F), int(n[s]), fc[s], mc[s], steps)
+        gi.sync()
+        t_seq += time.perf_counter() - t0
+    dE = np.abs(c1[:, 2] - c0[:, 2]) / np.abs(c0[:, 2])
+    scaleL = np.maximum(np.linalg.norm(c0[:, 6:9], axis=1), 1e-30)
+    dL = np.linalg.norm(c1[:, 6:9] - c0[:, 6:9], axis=1) / scaleL
+    return {
+        "sims": n_sims,
+        "particles": ntot,
+        "n_min": int(n.min()),
+        "n_max": int(n.max()),
+        "pairs_per_step": pairs,
+        "steps": steps,
+        "batched_sim_steps_per_s": n_sims * steps / t_batched,
+        "sequential_sim_steps_per_s": n_sims * steps / t_seq,
+        "speedup_vs_sequential": t_seq / t_batched,
+        "batched_pair_per_s": pairs * steps / t_batched,
+        "fp64_energy_rel_change_max": float(dE.max()),
+        "fp64_angmom_rel_change_max": float(dL.max()),
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -85,6 +139,9 @@ def main() -> int:
     ap.add_argument("--n-impactor", type=int, default=65536)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--ensemble-sims", type=int, default=64, help="0 skips the ensemble leg")
+    ap.add_argument("--ensemble-steps", type=int, default=100)
+    ap.add_argument("--only-ensemble", action="store_true", help="run the ensemble leg alone (profiling)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -101,6 +158,11 @@ def main() -> int:
         print("bench.py: no HIP device visible", file=sys.stderr)
         return 1
     lib = gi.load()
+
+    if args.only_ensemble:
+        out = ensemble_leg(gi, args.ensemble_sims, args.ensemble_steps)
+        print(json.dumps({"ensemble": out}), flush=True)
+        return 0
 
     nt, ni = args.n_target, args.n_impactor
     n = nt + ni
@@ -190,6 +252,10 @@ def main() -> int:
         },
         "parity": parity,
     }
+    if args.ensemble_sims > 0:
+        log(f"ensemble leg: {args.ensemble_sims} simulations x {args.ensemble_steps} steps")
+        result["ensemble"] = ensemble_leg(gi, args.ensemble_sims, args.ensemble_steps)
+        log(f"ensemble: {result['ensemble']}")
     if not args.no_cpu:
         log("timing the CPU baseline")
         result["cpu_baseline"] = cpu_baseline(pos, vel, fc, n, args.cpu_seconds)

@@ -90,7 +90,7 @@ struct Plan {
     Tiling tiling{4, 0, 0, 0, 0};
     int ntot = 0;
     int partial_len = 0;  // float4 elements
-    unsigned mask = 0;    // hash table size - 1
+    int buckets = 0;      // contact-grid buckets over all simulations
 };
 
 // Scratch and plan of one owner (the drop-in entry points of a device, or
@@ -98,12 +98,14 @@ struct Plan {
 struct Workspace {
     DevBuf<float4> partial;
     DevBuf<int> counts, starts, sorted, cellhash, block_sums;
+    DevBuf<float4> spos, svel;  // positions / velocities in bucket order
     DevBuf<GravTask> tasks;
     DevBuf<SimDesc> sims;
     DevBuf<int> block_sim, block_first, sim_first_block;
     DevBuf<double> moments, sums, pe;
     Plan plan;
     bool plan_valid = false;
+    bool batched = false;               // plans with the batched engine's tiling
     unsigned long long generation = 0;  // bumped on every re-plan
 };
 
@@ -178,25 +180,37 @@ static bool same_tiling(const Tiling& a, const Tiling& b) {
 }
 
 // Fill nchunk / partial_base of the simulations and build the task table.
-static int build_plan(std::vector<SimDesc> sims, const Tiling& tl, Plan* out) {
+// j-chunk length of one simulation.  It depends on that simulation's own size
+// only (through its row of the tuning table), never on the rest of a batch:
+// chunk boundaries fix the order of every float sum, so a simulation gives
+// bitwise the same forces alone and inside any ensemble.
+static long long sim_chunk_len(long long n) {
+    const Tiling t = choose_tiling(n, n * n, 1);
+    long long L = t.chunk_len;
+    if (L <= 0) {
+        const long long ib = 64LL * t.R;
+        L = ((n + ib - 1) / ib) * n / (t.target_tasks > 0 ? t.target_tasks : 1);
+    }
+    if (L < t.min_chunk) L = t.min_chunk;
+    if (L < 64) L = 64;
+    return (L + 63) / 64 * 64;
+}
+
+// batched: every simulation uses the fixed chunk length tl.chunk_len (the
+// batched engine); otherwise each uses its own size's standalone rule.
+static int build_plan(std::vector<SimDesc> sims, const Tiling& tl, bool batched, Plan* out) {
     Plan p;
     p.tiling = tl;
     long long ntot = 0;
-    long long work = 0;  // sum over active sims of iblocks * n
     const int ib = 64 * tl.R;
     for (auto& s : sims) {
         long long n = s.end - s.start;
         if (n < 0 || s.fe_end < s.start || s.fe_end > s.end || s.start != ntot)
             return set_error(GI_E_INVALID_ARG, "simulation ranges must be packed and ordered");
         ntot = s.end;
-        if (s.active) work += ((n + ib - 1) / ib) * n;
     }
     // The contact-grid scan handles up to 4096 * 1024 buckets = 2 * 2^21.
     if (ntot > (1LL << 21)) return set_error(GI_E_TOO_LARGE, "too many particles (%lld > 2^21)", ntot);
-    long long L = tl.chunk_len > 0 ? tl.chunk_len : work / (tl.target_tasks > 0 ? tl.target_tasks : 1);
-    if (L < tl.min_chunk) L = tl.min_chunk;
-    if (L < 64) L = 64;
-    L = (L + 63) / 64 * 64;
     long long pbase = 0;
     for (size_t si = 0; si < sims.size(); ++si) {
         SimDesc& s = sims[si];
@@ -204,6 +218,7 @@ static int build_plan(std::vector<SimDesc> sims, const Tiling& tl, Plan* out) {
         s.partial_base = (int)pbase;
         s.nchunk = 0;
         if (n == 0 || !s.active) continue;
+        const long long L = batched ? tl.chunk_len : sim_chunk_len(n);
         // j segments: iron [start, fe_end), silicate [fe_end, end); each split
         // into near-equal chunks of at most L.
         const int seg0[2] = {s.start, s.fe_end};
@@ -239,9 +254,18 @@ static int build_plan(std::vector<SimDesc> sims, const Tiling& tl, Plan* out) {
                      [](const GravTask& a, const GravTask& b) { return (a.j1 - a.j0) > (b.j1 - b.j0); });
     p.partial_len = (int)pbase;
     p.ntot = (int)ntot;
-    unsigned m = 1024;
-    while (m < 2u * (unsigned)ntot) m <<= 1;
-    p.mask = m - 1;
+    // Contact grid: each simulation owns a power-of-two range of buckets of at
+    // least its particle count (cells are visited by Morton code modulo it).
+    long long hb = 0;
+    for (auto& s : sims) {
+        unsigned sz = 64;
+        while (sz < (unsigned)(s.end - s.start)) sz <<= 1;
+        s.hbase = (int)hb;
+        s.hmask = sz - 1;
+        hb += sz;
+    }
+    if ((hb + 1023) / 1024 > 4096) return set_error(GI_E_TOO_LARGE, "contact grid too large (%lld buckets)", hb);
+    p.buckets = (int)hb;
     // moment blocks: 256 particles each, never straddling simulations
     p.sim_first_block.push_back(0);
     for (size_t si = 0; si < sims.size(); ++si) {
@@ -271,6 +295,10 @@ static int ensure_plan(Workspace* ws, const std::vector<SimDesc>& sims, hipStrea
     for (auto& s : sims)
         if (s.active) pairs += (long long)(s.end - s.start) * (s.end - s.start);
     Tiling tl = choose_tiling(ntot, pairs, (int)sims.size());
+    if (ws->batched) {
+        const BatchedTiling bt = batched_tiling();
+        tl = Tiling{bt.R, 0, bt.chunk_len, 0, 0};
+    }
     if (ws->plan_valid && same_sims(ws->plan.sims, sims) && same_tiling(ws->plan.tiling, tl)) {
         // same simulations; only partial_base/nchunk were filled in by the planner
         return 0;
@@ -283,19 +311,23 @@ static int ensure_plan(Workspace* ws, const std::vector<SimDesc>& sims, hipStrea
             SimDesc a = sims[k], b = ws->plan.sims[k];
             a.nchunk = b.nchunk;
             a.partial_base = b.partial_base;
+            a.hbase = b.hbase;
+            a.hmask = b.hmask;
             same = std::memcmp(&a, &b, sizeof a) == 0;
         }
         if (same) return 0;
     }
     Plan p;
-    int rc = build_plan(sims, tl, &p);
+    int rc = build_plan(sims, tl, ws->batched, &p);
     if (rc) return rc;
     if ((rc = ws->partial.ensure((size_t)p.partial_len + 1))) return rc;
-    if ((rc = ws->counts.ensure((size_t)p.mask + 1))) return rc;
-    if ((rc = ws->starts.ensure((size_t)p.mask + 2))) return rc;
+    if ((rc = ws->counts.ensure((size_t)p.buckets + 1))) return rc;
+    if ((rc = ws->starts.ensure((size_t)p.buckets + 2))) return rc;
+    if ((rc = ws->spos.ensure((size_t)p.ntot + 1))) return rc;
+    if ((rc = ws->svel.ensure((size_t)p.ntot + 1))) return rc;
     if ((rc = ws->sorted.ensure((size_t)p.ntot + 1))) return rc;
     if ((rc = ws->cellhash.ensure((size_t)p.ntot + 1))) return rc;
-    if ((rc = ws->block_sums.ensure(((size_t)p.mask + 1) / 1024 + 8))) return rc;
+    if ((rc = ws->block_sums.ensure((size_t)p.buckets / 1024 + 8))) return rc;
     if ((rc = upload(ws->tasks, p.tasks, st))) return rc;
     if ((rc = upload(ws->sims, p.sims, st))) return rc;
     if ((rc = upload(ws->block_sim, p.block_sim, st))) return rc;
@@ -317,13 +349,16 @@ static inline int nblk(long long n, int b) { return (int)((n + b - 1) / b); }
 // Launch sequences
 // ---------------------------------------------------------------------------
 
-static int launch_grid(Workspace* ws, const float4* pos, hipStream_t st) {
+// Contact grid of the current plan: bucket counts, offsets (scan), particle
+// ids per bucket sorted by id, and positions / velocities in that order
+// (vel may be null when only positions are needed).
+static int launch_grid(Workspace* ws, const float4* pos, const float4* vel, hipStream_t st) {
     const Plan& p = ws->plan;
-    const int m = (int)p.mask + 1;
+    const int m = p.buckets;
     const int nsims = (int)p.sims.size();
     hipLaunchKernelGGL(k_fill_int, dim3(nblk(m, 256)), dim3(256), 0, st, ws->counts.p, m, 0);
     hipLaunchKernelGGL(k_grid_count, dim3(nblk(p.ntot, 256)), dim3(256), 0, st, pos, p.ntot, ws->sims.p,
-                       nsims, p.mask, ws->counts.p, ws->cellhash.p);
+                       nsims, ws->counts.p, ws->cellhash.p);
     const int nb = nblk(m, 1024);
     hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(256), 0, st, ws->counts.p, m, ws->starts.p,
                        ws->block_sums.p);
@@ -333,6 +368,8 @@ static int launch_grid(Workspace* ws, const float4* pos, hipStream_t st) {
                        ws->starts.p, ws->counts.p, ws->sorted.p);
     hipLaunchKernelGGL(k_grid_sort_buckets, dim3(nblk(m, 256)), dim3(256), 0, st, m, ws->starts.p,
                        ws->sorted.p);
+    hipLaunchKernelGGL(k_grid_gather, dim3(nblk(p.ntot, 256)), dim3(256), 0, st, p.ntot,
+                       (const int*)ws->sorted.p, pos, vel, ws->spos.p, vel ? ws->svel.p : nullptr);
     GI_HIP(hipGetLastError());
     return 0;
 }
@@ -398,11 +435,12 @@ static int run_force(DevCtx* ctx, Workspace* ws, const float4* pos, const float4
                      hipStream_t st, bool timed) {
     const Plan& p = ws->plan;
     if (p.ntot == 0) return 0;
-    int rc = launch_grid(ws, pos, st);
+    int rc = launch_grid(ws, pos, vel, st);
     if (rc) return rc;
     if ((rc = launch_gravity(ctx, ws, pos, false, st, timed))) return rc;
-    hipLaunchKernelGGL(k_finish, dim3(nblk(p.ntot, 256)), dim3(256), 0, st, pos, vel, force, p.ntot, ws->sims.p,
-                       (int)p.sims.size(), ws->partial.p, ws->starts.p, ws->sorted.p, p.mask, p.tiling.variant);
+    hipLaunchKernelGGL(k_finish, dim3(nblk(p.ntot, 256)), dim3(256), 0, st, force, p.ntot, ws->sims.p,
+                       (int)p.sims.size(), ws->partial.p, ws->starts.p, ws->sorted.p, ws->spos.p, ws->svel.p,
+                       p.tiling.variant);
     GI_HIP(hipGetLastError());
     return 0;
 }
@@ -411,11 +449,11 @@ static int run_force(DevCtx* ctx, Workspace* ws, const float4* pos, const float4
 static int run_conserved(DevCtx* ctx, Workspace* ws, const float4* pos, const float4* vel, hipStream_t st) {
     const Plan& p = ws->plan;
     if (p.ntot == 0) return 0;
-    int rc = launch_grid(ws, pos, st);
+    int rc = launch_grid(ws, pos, nullptr, st);
     if (rc) return rc;
     if ((rc = launch_gravity(ctx, ws, pos, true, st, false))) return rc;
-    hipLaunchKernelGGL(k_finish_potential, dim3(nblk(p.ntot, 256)), dim3(256), 0, st, pos, p.ntot, ws->sims.p,
-                       (int)p.sims.size(), ws->partial.p, ws->starts.p, ws->sorted.p, p.mask, ws->pe.p);
+    hipLaunchKernelGGL(k_finish_potential, dim3(nblk(p.ntot, 256)), dim3(256), 0, st, p.ntot, ws->sims.p,
+                       (int)p.sims.size(), ws->partial.p, ws->starts.p, ws->sorted.p, ws->spos.p, ws->pe.p);
     const int nb = (int)p.block_sim.size();
     hipLaunchKernelGGL(k_moments, dim3(nb), dim3(256), 0, st, pos, vel, (const double*)ws->pe.p,
                        ws->block_sim.p, ws->block_first.p, ws->sims.p, 0.0, 0.0, ws->moments.p);
@@ -646,6 +684,17 @@ int gi_device_sync(void) {
 int gi_mem_info(size_t* free_bytes, size_t* total_bytes) {
     if (!free_bytes || !total_bytes) return set_error(GI_E_INVALID_ARG, "gi_mem_info: null pointer");
     GI_HIP(hipMemGetInfo(free_bytes, total_bytes));
+    return 0;
+}
+
+int gi_set_batched_tiling(int r, long long chunk_len) {
+    if (r == 0) {
+        batched_override() = BatchedTiling{0, 0};
+        return 0;
+    }
+    if ((r != 1 && r != 2 && r != 4 && r != 8) || chunk_len < 64)
+        return set_error(GI_E_INVALID_ARG, "gi_set_batched_tiling: r in {1,2,4,8} and chunk_len >= 64");
+    batched_override() = BatchedTiling{(chunk_len + 63) / 64 * 64, r};
     return 0;
 }
 
@@ -880,6 +929,8 @@ int gi_ensemble_create(gi_ensemble** out, int n_sims, const int* n, const gi_for
     GI_HIP(hipGetDevice(&dev));
     gi_ensemble* e = new gi_ensemble();
     e->device = dev;
+    e->ws.batched = true;
+    e->ws_diag.batched = true;
     int off = 0;
     for (int s = 0; s < n_sims; ++s) {
         if (n[s] <= 0 || fc[s].NFe < 0 || fc[s].NFe > n[s] || mc[s].NFe != fc[s].NFe) {

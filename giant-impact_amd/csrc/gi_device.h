@@ -18,12 +18,14 @@ struct PairLaw {
 // iron is [start, fe_end) (material by index, as the reference's id < NFe and
 // boarder tests).  Partial sums of the all-pairs pass for particle i and
 // j-chunk c live at partial[partial_base + c * (end - start) + (i - start)].
+// The contact grid of the simulation owns buckets [hbase, hbase + hmask + 1).
 struct SimDesc {
     int start, fe_end, end;
     int nchunk;
     int partial_base;
     int active;
-    int pad0, pad1;
+    int hbase;
+    unsigned hmask;
     PairLaw law;
     float dt, dt_over_mass_fe, dt_over_mass_si;
     float pad2;
@@ -40,11 +42,26 @@ struct GravTask {
 // half an ulp of every partial sum of r^2 and changes nothing.
 constexpr float kSelfEps = 1e-20f;
 
-// Hash of an integer cell (cell size 1 = the element diameter) and a
-// simulation id.
-__device__ __forceinline__ unsigned cell_hash(int cx, int cy, int cz, int sim) {
-    return ((unsigned)cx * 73856093u) ^ ((unsigned)cy * 19349663u) ^ ((unsigned)cz * 83492791u) ^
-           ((unsigned)sim * 2654435761u);
+// Spreads the low 10 bits of v to every third bit (bit k -> bit 3k).
+__device__ __forceinline__ unsigned spread3(unsigned v) {
+    v &= 0x3ffu;
+    v = (v | (v << 16)) & 0x030000ffu;
+    v = (v | (v << 8)) & 0x0300f00fu;
+    v = (v | (v << 4)) & 0x030c30c3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+// Bucket of an integer cell (cell size 1 = the element diameter) in the
+// simulation's own range of the contact grid: the Morton code of the cell,
+// so buckets that are close in memory hold cells that are close in space.
+// Distinct cells may share a bucket (codes taken modulo the range); readers
+// keep only the particles of the exact cell they look for.
+__device__ __forceinline__ unsigned cell_bucket(int hbase, unsigned hmask, int cx, int cy, int cz) {
+    const unsigned b = 1u << 20;  // bias: negative coordinates wrap like positive ones
+    const unsigned m = spread3((unsigned)cx + b) | (spread3((unsigned)cy + b) << 1) |
+                       (spread3((unsigned)cz + b) << 2);
+    return (unsigned)hbase + (m & hmask);
 }
 
 __device__ __forceinline__ int cell_coord(float x) {

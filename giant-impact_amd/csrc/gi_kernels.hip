@@ -144,13 +144,12 @@ __device__ __forceinline__ int find_sim(const SimDesc* __restrict__ sims, int ns
 }
 
 __global__ void k_grid_count(const float4* __restrict__ pos, int ntot, const SimDesc* __restrict__ sims,
-                             int nsims, unsigned mask, int* __restrict__ counts,
-                             int* __restrict__ cellhash) {
+                             int nsims, int* __restrict__ counts, int* __restrict__ cellhash) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ntot) return;
-    int s = find_sim(sims, nsims, i);
+    const SimDesc& sd = sims[find_sim(sims, nsims, i)];
     float4 p = pos[i];
-    unsigned h = cell_hash(cell_coord(p.x), cell_coord(p.y), cell_coord(p.z), s) & mask;
+    unsigned h = cell_bucket(sd.hbase, sd.hmask, cell_coord(p.x), cell_coord(p.y), cell_coord(p.z));
     cellhash[i] = (int)h;
     atomicAdd(&counts[h], 1);
 }
@@ -246,20 +245,34 @@ __global__ void k_grid_sort_buckets(int m, const int* __restrict__ starts, int* 
     }
 }
 
+// Positions and velocities in bucket order, so the contact pass reads the
+// particles of a cell from consecutive memory.
+__global__ void k_grid_gather(int ntot, const int* __restrict__ sorted, const float4* __restrict__ pos,
+                              const float4* __restrict__ vel, float4* __restrict__ spos,
+                              float4* __restrict__ svel) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntot) return;
+    const int i = sorted[t];
+    spos[t] = pos[i];
+    if (svel) svel[t] = vel[i];
+}
+
 // ---------------------------------------------------------------------------
 // Pass 3: chunk sums + exact contact correction
 // ---------------------------------------------------------------------------
 
-__global__ __launch_bounds__(256) void k_finish(const float4* __restrict__ pos,
-                                                const float4* __restrict__ vel,
-                                                float4* __restrict__ force, int ntot,
+__global__ __launch_bounds__(256) void k_finish(float4* __restrict__ force, int ntot,
                                                 const SimDesc* __restrict__ sims, int nsims,
                                                 const float4* __restrict__ partial,
                                                 const int* __restrict__ starts,
-                                                const int* __restrict__ sorted, unsigned mask,
-                                                int variant) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= ntot) return;
+                                                const int* __restrict__ sorted,
+                                                const float4* __restrict__ spos,
+                                                const float4* __restrict__ svel, int variant) {
+    // Threads walk the particles in bucket (Morton) order: the lanes of a
+    // wavefront look up the same and adjacent cells.
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntot) return;
+    const int i = sorted[t];
     const int s = find_sim(sims, nsims, i);
     const SimDesc& sd = sims[s];
     if (!sd.active) return;
@@ -276,26 +289,29 @@ __global__ __launch_bounds__(256) void k_finish(const float4* __restrict__ pos,
     fy -= ky;
     fz -= kz;
 
-    const float4 pi = pos[i];
-    const float4 vi = vel[i];
+    const float4 pi = spos[t];
+    const float4 vi = svel[t];
     const int ccx = cell_coord(pi.x), ccy = cell_coord(pi.y), ccz = cell_coord(pi.z);
     const bool ife = i < sd.fe_end;
     // Contact terms are rounded exactly as the reference rounds them (float
     // products of a float magnitude); only their sum is carried in FP64, so
     // cancellation between large contact forces adds no summation error.
+    // Cells are visited in a fixed order and a bucket lists its particles by
+    // index, so the order of the terms does not depend on the rest of the
+    // launch.
     double ex = 0.0, ey = 0.0, ez = 0.0;
     for (int oz = -1; oz <= 1; ++oz)
         for (int oy = -1; oy <= 1; ++oy)
             for (int ox = -1; ox <= 1; ++ox) {
                 const int qx = ccx + ox, qy = ccy + oy, qz = ccz + oz;
-                const unsigned h = cell_hash(qx, qy, qz, s) & mask;
+                const unsigned h = cell_bucket(sd.hbase, sd.hmask, qx, qy, qz);
                 const int b0 = starts[h], b1 = starts[h + 1];
                 for (int k = b0; k < b1; ++k) {
-                    const int j = sorted[k];
-                    if (j == i || j < sd.start || j >= sd.end) continue;
-                    const float4 pj = pos[j];
+                    const float4 pj = spos[k];
                     if (cell_coord(pj.x) != qx || cell_coord(pj.y) != qy || cell_coord(pj.z) != qz)
                         continue;  // another cell sharing the bucket
+                    const int j = sorted[k];
+                    if (j == i || j < sd.start || j >= sd.end) continue;
                     const float dpx = pj.x - pi.x;
                     const float dpy = pj.y - pi.y;
                     const float dpz = pj.z - pi.z;
@@ -304,7 +320,7 @@ __global__ __launch_bounds__(256) void k_finish(const float4* __restrict__ pos,
                     if (1.0f <= r) continue;
                     const float invr = 1.0f / r;
                     const int test = (int)ife + (int)(j < sd.fe_end);
-                    const float4 vj = vel[j];
+                    const float4 vj = svel[k];
                     const float dvx = vj.x - vi.x, dvy = vj.y - vi.y, dvz = vj.z - vi.z;
                     const float dot = dpx * dvx + dpy * dvy + dpz * dvz;
                     const float fm = contact_pair_mag(sd.law, test, r, r2, dot);
@@ -409,35 +425,36 @@ __global__ void k_move_damped_separate(float4* __restrict__ pos, float4* __restr
 // Per-particle potential: -(1/2) * (sum of the chunk partials of 1/r scaled
 // by G mi mj) plus, for pairs with r < 1, half the difference between the
 // exact contact potential and the gravity term pass 1 added.  FP64 output.
-__global__ __launch_bounds__(256) void k_finish_potential(const float4* __restrict__ pos, int ntot,
-                                                          const SimDesc* __restrict__ sims,
+__global__ __launch_bounds__(256) void k_finish_potential(int ntot, const SimDesc* __restrict__ sims,
                                                           int nsims,
                                                           const float4* __restrict__ partial,
                                                           const int* __restrict__ starts,
                                                           const int* __restrict__ sorted,
-                                                          unsigned mask, double* __restrict__ pe) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= ntot) return;
+                                                          const float4* __restrict__ spos,
+                                                          double* __restrict__ pe) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntot) return;
+    const int i = sorted[t];
     const int s = find_sim(sims, nsims, i);
     const SimDesc& sd = sims[s];
     const int n = sd.end - sd.start;
     double phi = 0.0;
     for (int c = 0; c < sd.nchunk; ++c) phi += (double)partial[sd.partial_base + c * n + (i - sd.start)].x;
     double u = -phi;
-    const float4 pi = pos[i];
+    const float4 pi = spos[t];
     const int ccx = cell_coord(pi.x), ccy = cell_coord(pi.y), ccz = cell_coord(pi.z);
     const bool ife = i < sd.fe_end;
     for (int oz = -1; oz <= 1; ++oz)
         for (int oy = -1; oy <= 1; ++oy)
             for (int ox = -1; ox <= 1; ++ox) {
                 const int qx = ccx + ox, qy = ccy + oy, qz = ccz + oz;
-                const unsigned h = cell_hash(qx, qy, qz, s) & mask;
+                const unsigned h = cell_bucket(sd.hbase, sd.hmask, qx, qy, qz);
                 for (int k = starts[h]; k < starts[h + 1]; ++k) {
-                    const int j = sorted[k];
-                    if (j == i || j < sd.start || j >= sd.end) continue;
-                    const float4 pj = pos[j];
+                    const float4 pj = spos[k];
                     if (cell_coord(pj.x) != qx || cell_coord(pj.y) != qy || cell_coord(pj.z) != qz)
                         continue;
+                    const int j = sorted[k];
+                    if (j == i || j < sd.start || j >= sd.end) continue;
                     const double dx = (double)pj.x - pi.x, dy = (double)pj.y - pi.y,
                                  dz = (double)pj.z - pi.z;
                     const double r = sqrt(dx * dx + dy * dy + dz * dz);

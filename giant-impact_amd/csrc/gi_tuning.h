@@ -81,6 +81,48 @@ inline const std::vector<TuningRow>& tuning_table() {
     return rows;
 }
 
+// Tiling of the batched engine (ensembles and the request loop): one j-chunk
+// length for every simulation, whatever its size or company, so that a
+// simulation's sums are the same in any batch.  Row "B chunk_len R" of the
+// tuning file; gi_set_batched_tiling overrides it.
+struct BatchedTiling {
+    long long chunk_len;
+    int R;
+};
+
+inline BatchedTiling& batched_override() {
+    static BatchedTiling b{0, 0};
+    return b;
+}
+
+inline BatchedTiling batched_tiling() {
+    if (batched_override().chunk_len > 0) return batched_override();
+    static BatchedTiling from_file{0, 0};
+    static std::once_flag once;
+    std::call_once(once, [] {
+        Dl_info info;
+        if (!dladdr((void*)&tuning_anchor, &info) || !info.dli_fname) return;
+        std::string path(info.dli_fname);
+        size_t cut = path.rfind('/');
+        if (cut == std::string::npos) return;
+        path = path.substr(0, cut);
+        cut = path.rfind('/');
+        path = (cut == std::string::npos ? std::string(".") : path.substr(0, cut)) + "/tuning/gfx950.txt";
+        FILE* f = std::fopen(path.c_str(), "r");
+        if (!f) return;
+        char line[256];
+        while (std::fgets(line, sizeof line, f)) {
+            long long L;
+            int R;
+            if (std::sscanf(line, "B %lld %d", &L, &R) == 2 && L >= 64 && (R == 1 || R == 2 || R == 4 || R == 8))
+                from_file = BatchedTiling{L, R};
+        }
+        std::fclose(f);
+    });
+    if (from_file.chunk_len > 0) return from_file;
+    return BatchedTiling{4096, 1};
+}
+
 inline Tiling choose_tiling(long long ntot, long long pairs, int nsims) {
     (void)pairs;
     (void)nsims;

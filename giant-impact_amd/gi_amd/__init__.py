@@ -251,16 +251,14 @@ def load() -> C.CDLL:
             i,
             [vp, vp, vp, i, P(ForceCollisionConstants), P(MoveCollisionConstants), i, i, P(C.c_double), P(C.c_double), vp],
         ),
-        "gi_ensemble_create": (
-            i,
-            [P(vp), i, P(i), P(ForceCollisionConstants), P(MoveCollisionConstants), vp, vp, vp],
-        ),
+        "gi_ensemble_create": (i, [P(vp), i, P(i), vp, vp, vp, vp, vp]),
         "gi_ensemble_destroy": (i, [vp]),
         "gi_ensemble_steps": (i, [vp, i]),
         "gi_ensemble_set_active": (i, [vp, P(i)]),
         "gi_ensemble_conserved": (i, [vp, P(C.c_double)]),
         "gi_ensemble_work": (i, [vp, P(C.c_longlong), P(C.c_double)]),
         "gi_set_tiling": (i, [i, i, C.c_longlong, C.c_longlong, C.c_longlong]),
+        "gi_set_batched_tiling": (i, [i, C.c_longlong]),
     }
     for name, (res, args) in _optional.items():
         if hasattr(lib, name):
@@ -426,3 +424,108 @@ def move_damped_separate(pos, vel, force, n: int, mc: MoveSeparateConstants, d1:
 
 def sync() -> None:
     call("gi_device_sync")
+
+
+# ---------------------------------------------------------------------------
+# Batched ensemble
+# ---------------------------------------------------------------------------
+
+
+def ensemble_configs(n_sims: int, seed: int = 20261205, n_range=(4096, 32768), impactor_frac=(0.1, 0.3),
+                     angle=(30.0, 60.0), speed=(1.0, 1.4), spin=(0.0, 0.6), log_uniform_n: bool = False) -> list:
+    """Seeded impact parameters of an uneven ensemble (one dict per simulation)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for s in range(n_sims):
+        if log_uniform_n:
+            n = int(round(np.exp(rng.uniform(np.log(n_range[0]), np.log(n_range[1])))))
+        else:
+            n = int(rng.integers(n_range[0], n_range[1] + 1))
+        f = float(rng.uniform(*impactor_frac))
+        ni = max(1, int(round(f * n)))
+        out.append(
+            dict(
+                n_target=n - ni,
+                n_impactor=ni,
+                seed=int(seed * 1000 + s),
+                angle_deg=float(rng.uniform(*angle)),
+                speed_vesc=float(rng.uniform(*speed)),
+                target_spin=float(rng.uniform(*spin)),
+            )
+        )
+    return out
+
+
+def make_ensemble(configs: list):
+    """Pack the impacts of `configs` back to back.
+
+    Returns (pos, vel, n (int32 array), fc array, mc array, infos)."""
+    parts = [make_impact(**c) for c in configs]
+    pos = np.concatenate([p[0] for p in parts])
+    vel = np.concatenate([p[1] for p in parts])
+    n = np.array([len(p[0]) for p in parts], dtype=np.int32)
+    fc = (ForceCollisionConstants * len(parts))(*[p[2] for p in parts])
+    mc = (MoveCollisionConstants * len(parts))(*[p[3] for p in parts])
+    return pos, vel, n, fc, mc, [p[4] for p in parts]
+
+
+class Ensemble:
+    """gi_ensemble_*: many impacts stepped together, one launch per kernel per step."""
+
+    def __init__(self, P: DeviceArray, V: DeviceArray, n, fc, mc, stream=None):
+        self.n = np.ascontiguousarray(n, dtype=np.int32)
+        self.n_sims = len(self.n)
+        self.offsets = np.concatenate([[0], np.cumsum(self.n)]).astype(np.int64)
+        self.handle = C.c_void_p()
+        call(
+            "gi_ensemble_create",
+            C.byref(self.handle),
+            self.n_sims,
+            self.n.ctypes.data_as(C.POINTER(C.c_int)),
+            C.cast(fc, C.c_void_p),
+            C.cast(mc, C.c_void_p),
+            P.ptr,
+            V.ptr,
+            _p(stream),
+        )
+
+    def steps(self, k: int) -> None:
+        call("gi_ensemble_steps", self.handle, int(k))
+
+    def set_active(self, active) -> None:
+        a = np.ascontiguousarray(active, dtype=np.int32)
+        call("gi_ensemble_set_active", self.handle, a.ctypes.data_as(C.POINTER(C.c_int)))
+
+    def conserved(self) -> np.ndarray:
+        out = np.zeros((self.n_sims, 9), dtype=np.float64)
+        call("gi_ensemble_conserved", self.handle, out.ctypes.data_as(C.POINTER(C.c_double)))
+        return out
+
+    def work(self):
+        p, q = C.c_longlong(), C.c_double()
+        call("gi_ensemble_work", self.handle, C.byref(p), C.byref(q))
+        return p.value, q.value
+
+    def close(self) -> None:
+        if self.handle is not None and self.handle.value:
+            load().gi_ensemble_destroy(self.handle)
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def impact_steps(P, V, F, n: int, fc, mc, steps: int, recenter_rate: int = 0, com3=None, vel3=None, stream=None):
+    c3 = (C.c_double * 3)(*com3) if com3 is not None else None
+    v3 = (C.c_double * 3)(*vel3) if vel3 is not None else None
+    call("gi_impact_steps", _p(P), _p(V), _p(F), int(n), C.byref(fc), C.byref(mc), int(steps), int(recenter_rate),
+         c3, v3, _p(stream))
+
+
+def conserved_quantities(P, V, n: int, fc, stream=None) -> np.ndarray:
+    out = (C.c_double * 9)()
+    call("gi_conserved_quantities", _p(P), _p(V), int(n), C.byref(fc), out, _p(stream))
+    return np.array(list(out))

@@ -56,12 +56,44 @@ def time_gravity(n_target: int, n_impactor: int, reps: int):
     return run, n
 
 
+def tune_batched(quick: bool):
+    """Chunk length and R of the batched engine, timed on the 64-simulation
+    uneven ensemble (N uniform in [4096, 32768])."""
+    from . import Ensemble, ensemble_configs, make_ensemble
+
+    pos, vel, n, fc, mc, _ = make_ensemble(ensemble_configs(64))
+    ntot = int(n.sum())
+    pairs = float((n.astype(np.float64) ** 2).sum())
+    P, V = DeviceArray(ntot, pos), DeviceArray(ntot, vel)
+    best = None
+    Ls = [1024, 2048, 4096, 8192, 16384] if not quick else [4096]
+    for R, L in itertools.product([1, 2, 4], Ls):
+        call("gi_set_batched_tiling", R, L)
+        P.upload(pos)
+        V.upload(vel)
+        e = Ensemble(P, V, n, fc, mc)
+        e.steps(1)
+        sync()
+        t0 = time.perf_counter()
+        e.steps(10)
+        sync()
+        dt = (time.perf_counter() - t0) / 10
+        e.close()
+        print(f"batched R={R} L={L:6d}: {dt * 1e3:8.3f} ms/step  {pairs / dt:.3e} pair/s", flush=True)
+        if best is None or dt < best[0]:
+            best = (dt, L, R)
+    call("gi_set_batched_tiling", 0, 0)
+    print(f"# batched: best chunk_len={best[1]} R={best[2]} ({pairs / best[0]:.3e} pair/s)", flush=True)
+    return best[1], best[2]
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--quick", action="store_true", help="fewer candidates (smoke run)")
     ap.add_argument("--out", default=str(TUNING_DIR / "gfx950.txt"))
     ap.add_argument("--max-class", type=int, default=len(CLASSES))
     ap.add_argument("--variants", action="store_true", help="also sweep the rsq*rcp variant")
+    ap.add_argument("--batched", action="store_true", help="also tune the batched engine (64-simulation ensemble)")
     args = ap.parse_args(argv)
     if device_count() < 1:
         print("autotune: no HIP device", file=sys.stderr)
@@ -87,6 +119,7 @@ def main(argv=None) -> int:
         print(f"# class n<={n_max}: best R={R} variant={v} tasks={tt} ({n * n / (ms / 1e3):.3e} pair/s)", flush=True)
         rows.append((n_max, R, v, 0, tt, 128 if n < 4096 else 512))
     call("gi_set_tiling", 0, 0, 0, 0, 0)
+    batched = tune_batched(args.quick) if args.batched else None
     out = Path(args.out)
     out.parent.mkdir(parents=True, exist_ok=True)
     with open(out, "w") as f:
@@ -94,6 +127,9 @@ def main(argv=None) -> int:
         f.write("# n_max R variant chunk_len target_tasks min_chunk\n")
         for r in rows:
             f.write(" ".join(str(x) for x in r) + "\n")
+        if batched:
+            f.write("# batched engine: B chunk_len R\n")
+            f.write(f"B {batched[0]} {batched[1]}\n")
     print(f"wrote {out}")
     return 0
 

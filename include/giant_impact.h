@@ -247,6 +247,14 @@ int gi_ensemble_work(gi_ensemble *e, long long *particles, double *pairs);
  * rebuilt on their next use. */
 int gi_set_tiling(int r, int variant, long long chunk_len, long long target_tasks, long long min_chunk);
 
+/* Tiling of the batched engine (ensembles, request loop): every simulation's
+ * j-range is cut into chunks of chunk_len whatever the batch, so a simulation
+ * computes bitwise the same alone (an ensemble of one) and in any mix; r =
+ * i-particles per lane of the launch.  Default: row "B chunk_len r" of the
+ * tuning file.  r = 0 restores the default; takes effect at the next re-plan
+ * (a new ensemble or a change of the active set). */
+int gi_set_batched_tiling(int r, long long chunk_len);
+
 /* ---- kernel timing (HIP events around the all-pairs force kernel) ------- */
 
 /* Enable/disable per-launch timing (HIP events on the launching stream) of the

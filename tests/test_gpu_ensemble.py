@@ -1,0 +1,121 @@
+"""The batched ensemble: many impacts of uneven N stepped in one launch per
+kernel, captured in a HIP graph.  Each simulation's arithmetic depends only on
+itself (fixed chunk length of the batched engine), so its results are bitwise
+the same alone (an ensemble of one) and in any mix."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from conftest import f4
+
+pytestmark = pytest.mark.gpu
+
+
+def _ensemble_of_one(gi, pos, vel, fc, mc, steps):
+    n = len(pos)
+    P, V = gi.DeviceArray(n, pos), gi.DeviceArray(n, vel)
+    e = gi.Ensemble(P, V, np.array([n], np.int32), (gi.ForceCollisionConstants * 1)(fc),
+                    (gi.MoveCollisionConstants * 1)(mc))
+    e.steps(steps)
+    gi.sync()
+    out = P.download(), V.download()
+    e.close()
+    return out
+
+
+def _plain(gi, pos, vel, fc, mc, steps):
+    n = len(pos)
+    P, V, F = gi.DeviceArray(n, pos), gi.DeviceArray(n, vel), gi.DeviceArray(n, f4(n))
+    gi.impact_steps(P, V, F, n, fc, mc, steps)
+    gi.sync()
+    return P.download(), V.download()
+
+
+def test_ensemble_result_is_bitwise_the_same_alone_and_in_the_mix(gpu):
+    cfg = gpu.ensemble_configs(5, seed=3, n_range=(700, 4100))
+    pos, vel, n, fc, mc, _ = gpu.make_ensemble(cfg)
+    print("n =", list(n), flush=True)
+    P, V = gpu.DeviceArray(len(pos), pos), gpu.DeviceArray(len(pos), vel)
+    ens = gpu.Ensemble(P, V, n, fc, mc)
+    ens.steps(3)
+    ens.steps(4)
+    gpu.sync()
+    p_all, v_all = P.download(), V.download()
+    off = ens.offsets
+    for s in range(len(n)):
+        a, b = off[s], off[s + 1]
+        p1, v1 = _ensemble_of_one(gpu, pos[a:b], vel[a:b], fc[s], mc[s], 7)
+        assert np.array_equal(p_all[a:b], p1), s
+        assert np.array_equal(v_all[a:b], v1), s
+        # the plain single-simulation path agrees to float rounding
+        p2, v2 = _plain(gpu, pos[a:b], vel[a:b], fc[s], mc[s], 7)
+        assert np.abs(v2[:, :3] - v1[:, :3]).max() <= 1e-4 * np.abs(v1[:, :3]).max()
+        assert np.abs(p2[:, :3] - p1[:, :3]).max() <= 1e-5 * np.abs(p1[:, :3]).max()
+    ens.close()
+
+
+def test_inactive_simulations_are_frozen(gpu):
+    cfg = gpu.ensemble_configs(4, seed=4, n_range=(600, 2000))
+    pos, vel, n, fc, mc, _ = gpu.make_ensemble(cfg)
+    P, V = gpu.DeviceArray(len(pos), pos), gpu.DeviceArray(len(pos), vel)
+    ens = gpu.Ensemble(P, V, n, fc, mc)
+    ens.set_active([1, 0, 1, 0])
+    ens.steps(5)
+    ens.set_active([1, 1, 1, 1])
+    ens.steps(2)
+    gpu.sync()
+    p_all, v_all = P.download(), V.download()
+    off = ens.offsets
+    for s, k in zip(range(4), (7, 2, 7, 2)):
+        a, b = off[s], off[s + 1]
+        p1, v1 = _ensemble_of_one(gpu, pos[a:b], vel[a:b], fc[s], mc[s], k)
+        assert np.array_equal(p_all[a:b], p1) and np.array_equal(v_all[a:b], v1), s
+    work = ens.work()
+    assert work[0] == int(n.sum()) and work[1] == float((n.astype(np.float64) ** 2).sum())
+    ens.close()
+
+
+def test_ensemble_force_parity_against_the_oracle(gpu, oracle):
+    """One batched step from a known state: the velocity change of every
+    simulation is dt/m times its force, which must match the FP64 oracle."""
+    from parity import assert_force_parity, force_metrics
+
+    cfg = gpu.ensemble_configs(3, seed=8, n_range=(800, 2500))
+    pos, vel, n, fc, mc, _ = gpu.make_ensemble(cfg)
+    P, V = gpu.DeviceArray(len(pos), pos), gpu.DeviceArray(len(pos), vel)
+    ens = gpu.Ensemble(P, V, n, fc, mc)
+    ens.steps(1)
+    gpu.sync()
+    v1 = V.download()
+    off = ens.offsets
+    for s in range(3):
+        a, b = off[s], off[s + 1]
+        t = np.where(np.arange(b - a) < fc[s].NFe, np.float32(mc[s].DtOverMassFe), np.float32(mc[s].DtOverMassSi))
+        F_est = (v1[a:b, :3].astype(np.float64) - vel[a:b, :3]) / t[:, None]
+        F64, cond, near = oracle.force_collision_f64(pos[a:b], vel[a:b], fc[s])
+        met = force_metrics(F_est, F64, cond, near)
+        # v + f*t rounds once in float: allow that rounding on top of the force tolerance
+        slack = np.abs(vel[a:b, :3]).max() * 2 ** -23 / t.min()
+        err = np.linalg.norm(F_est - F64, axis=1)
+        assert np.all(err[~near] <= 1e-5 * cond[~near] + slack), (s, met)
+    ens.close()
+
+
+def test_ensemble_conserved_quantities_match_single(gpu):
+    cfg = gpu.ensemble_configs(3, seed=5, n_range=(900, 2500))
+    pos, vel, n, fc, mc, _ = gpu.make_ensemble(cfg)
+    P, V = gpu.DeviceArray(len(pos), pos), gpu.DeviceArray(len(pos), vel)
+    ens = gpu.Ensemble(P, V, n, fc, mc)
+    c = ens.conserved()
+    off = ens.offsets
+    for s in range(3):
+        a, b = off[s], off[s + 1]
+        Ps, Vs = gpu.DeviceArray(b - a, pos[a:b]), gpu.DeviceArray(b - a, vel[a:b])
+        one = gpu.conserved_quantities(Ps, Vs, b - a, fc[s])
+        # kinetic energy, momentum, angular momentum: FP64 sums of the same terms
+        idx = [0, 3, 4, 5, 6, 7, 8]
+        assert np.allclose(c[s][idx], one[idx], rtol=1e-12, atol=1e-12 * np.abs(one).max()), s
+        # the potential's float chunk sums are cut differently by the two engines
+        assert np.allclose(c[s][1:3], one[1:3], rtol=2e-6), s
+    ens.close()
