@@ -259,6 +259,7 @@ def load() -> C.CDLL:
         "gi_ensemble_work": (i, [vp, P(C.c_longlong), P(C.c_double)]),
         "gi_set_tiling": (i, [i, i, C.c_longlong, C.c_longlong, C.c_longlong]),
         "gi_set_batched_tiling": (i, [i, C.c_longlong]),
+        "gi_serve_run": (i, [vp, i, vp, vp, vp]),
     }
     for name, (res, args) in _optional.items():
         if hasattr(lib, name):
@@ -529,3 +530,101 @@ def conserved_quantities(P, V, n: int, fc, stream=None) -> np.ndarray:
     out = (C.c_double * 9)()
     call("gi_conserved_quantities", _p(P), _p(V), int(n), C.byref(fc), out, _p(stream))
     return np.array(list(out))
+
+
+# ---------------------------------------------------------------------------
+# Request loop
+# ---------------------------------------------------------------------------
+
+
+class ServeRequest(C.Structure):
+    _fields_ = [
+        ("impact", ImpactConfig),
+        ("steps", C.c_int),
+        ("arrival_s", C.c_double),
+        ("final_pos", C.c_void_p),
+        ("final_vel", C.c_void_p),
+    ]
+
+
+class ServeResult(C.Structure):
+    _fields_ = [
+        ("n", C.c_int),
+        ("steps", C.c_int),
+        ("arrival_s", C.c_double),
+        ("admit_s", C.c_double),
+        ("finish_s", C.c_double),
+        ("time_to_result_s", C.c_double),
+        ("energy0", C.c_double),
+        ("energy1", C.c_double),
+        ("angmom0", C.c_double * 3),
+        ("angmom1", C.c_double * 3),
+        ("energy_rel_change", C.c_double),
+        ("angmom_rel_change", C.c_double),
+    ]
+
+
+class ServeOptions(C.Structure):
+    _fields_ = [
+        ("max_particles", C.c_longlong),
+        ("max_sims", C.c_int),
+        ("quantum_steps", C.c_int),
+        ("hbm_fraction", C.c_double),
+    ]
+
+
+class ServeSummary(C.Structure):
+    _fields_ = [
+        ("completed", C.c_int),
+        ("wall_s", C.c_double),
+        ("total_sim_steps", C.c_double),
+        ("sim_steps_per_s", C.c_double),
+        ("pair_interactions_per_s", C.c_double),
+        ("p50_time_to_result_s", C.c_double),
+        ("p99_time_to_result_s", C.c_double),
+        ("max_time_to_result_s", C.c_double),
+        ("capacity_particles", C.c_longlong),
+        ("peak_resident_particles", C.c_longlong),
+        ("peak_resident_sims", C.c_int),
+        ("rounds", C.c_int),
+        ("max_energy_rel_change", C.c_double),
+        ("max_angmom_rel_change", C.c_double),
+    ]
+
+
+def _struct_dict(s) -> dict:
+    out = {}
+    for name, _ in s._fields_:
+        v = getattr(s, name)
+        out[name] = list(v) if isinstance(v, C.Array) else v
+    return out
+
+
+def serve(requests: list, *, max_particles: int = 0, max_sims: int = 0, quantum_steps: int = 0,
+          hbm_fraction: float = 0.0, keep_final: bool = False):
+    """Run the request loop.  requests: dicts with the make_impact keywords plus
+    'steps' and 'arrival_s'.  Returns (results, summary[, finals])."""
+    k = len(requests)
+    reqs = (ServeRequest * k)()
+    finals = []
+    for i, r in enumerate(requests):
+        cfg = ImpactConfig(
+            r["n_target"], r["n_impactor"], r.get("fe_fraction", 0.3), r.get("jitter", 0.1), r.get("seed", 1),
+            r.get("angle_deg", 45.0), r.get("speed_vesc", 1.2), r.get("target_spin", 0.0),
+            r.get("impactor_spin", 0.0), r.get("gap_in_target_radii", 1.0),
+        )
+        reqs[i].impact = cfg
+        reqs[i].steps = int(r["steps"])
+        reqs[i].arrival_s = float(r.get("arrival_s", 0.0))
+        if keep_final:
+            n = r["n_target"] + r["n_impactor"]
+            fp, fv = np.zeros((n, 4), np.float32), np.zeros((n, 4), np.float32)
+            finals.append((fp, fv))
+            reqs[i].final_pos = fp.ctypes.data
+            reqs[i].final_vel = fv.ctypes.data
+    res = (ServeResult * k)()
+    summ = ServeSummary()
+    opt = ServeOptions(max_particles, max_sims, quantum_steps, hbm_fraction)
+    call("gi_serve_run", reqs, k, C.byref(opt), res, C.byref(summ))
+    out = ([_struct_dict(x) for x in res], _struct_dict(summ))
+    return out + (finals,) if keep_final else out

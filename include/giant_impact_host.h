@@ -97,6 +97,57 @@ int gi_make_impact(const gi_impact_config *cfg, gi_float4 *pos, gi_float4 *vel,
                    gi_force_collision_constants *fc, gi_move_collision_constants *mc,
                    gi_impact_info *info);
 
+/* ---- request loop (continuous batching of impact-sweep jobs) ------------- */
+
+typedef struct gi_serve_request {
+    gi_impact_config impact; /* the impact to simulate (seeded synthetic bodies) */
+    int steps;               /* leapfrog steps to run */
+    double arrival_s;        /* arrival time in seconds after the start of the run */
+    gi_float4 *final_pos;    /* optional host buffers (n records) for the final state */
+    gi_float4 *final_vel;
+} gi_serve_request;
+
+typedef struct gi_serve_result {
+    int n;                  /* particles */
+    int steps;              /* steps run */
+    double arrival_s, admit_s, finish_s; /* seconds after the start of the run */
+    double time_to_result_s;             /* finish_s - arrival_s */
+    double energy0, energy1;             /* FP64 total energy at admission / completion */
+    double angmom0[3], angmom1[3];       /* FP64 angular momentum about the centre of mass */
+    double energy_rel_change;            /* |E1 - E0| / |E0| */
+    double angmom_rel_change;            /* |L1 - L0| / |L0| */
+} gi_serve_result;
+
+typedef struct gi_serve_options {
+    long long max_particles; /* resident particle capacity; 0 = from free HBM (capped at 2^21) */
+    int max_sims;            /* resident simulation slots; 0 = 256 */
+    int quantum_steps;       /* steps per scheduling round; 0 = 100 */
+    double hbm_fraction;     /* share of free HBM the loop may use; 0 = 0.5 */
+} gi_serve_options;
+
+typedef struct gi_serve_summary {
+    int completed;
+    double wall_s;
+    double total_sim_steps;       /* sum over requests of steps */
+    double sim_steps_per_s;       /* total_sim_steps / wall_s */
+    double pair_interactions_per_s;
+    double p50_time_to_result_s, p99_time_to_result_s, max_time_to_result_s;
+    long long capacity_particles; /* admission limit used */
+    long long peak_resident_particles;
+    int peak_resident_sims;
+    int rounds;                   /* scheduling rounds (batched launches of quantum steps) */
+    double max_energy_rel_change, max_angmom_rel_change;
+} gi_serve_summary;
+
+/* Serve `n_requests` requests on the current device: requests are admitted in
+ * arrival order as soon as they have arrived and their particles fit in the
+ * free capacity and a slot is free; every resident request advances in the
+ * same batched launches (continuous batching); a request leaves as soon as
+ * its steps are done, freeing room for the next.  results has n_requests
+ * entries (request order).  Blocking; returns when every request is done. */
+int gi_serve_run(const gi_serve_request *reqs, int n_requests, const gi_serve_options *opt,
+                 gi_serve_result *results, gi_serve_summary *summary);
+
 #ifdef __cplusplus
 }
 #endif
