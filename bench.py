@@ -142,6 +142,8 @@ def main() -> int:
     ap.add_argument("--ensemble-sims", type=int, default=64, help="0 skips the ensemble leg")
     ap.add_argument("--ensemble-steps", type=int, default=100)
     ap.add_argument("--only-ensemble", action="store_true", help="run the ensemble leg alone (profiling)")
+    ap.add_argument("--profile-probe", action="store_true",
+                    help="two steps + two 64 MiB copies, for hardware-counter passes")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -162,6 +164,27 @@ def main() -> int:
     if args.only_ensemble:
         out = ensemble_leg(gi, args.ensemble_sims, args.ensemble_steps)
         print(json.dumps({"ensemble": out}), flush=True)
+        return 0
+
+    if args.profile_probe:
+        # Short fixed workload for hardware-counter passes: two force + move
+        # steps of the benchmark impact, then two 64 MiB device copies whose
+        # known byte counts calibrate FETCH_SIZE / WRITE_SIZE.
+        pos, vel, fc, mc, _ = gi.make_impact(args.n_target, args.n_impactor, seed=20261202 + 2, angle_deg=45.0,
+                                             speed_vesc=1.1)
+        n = len(pos)
+        P, V, F = gi.DeviceArray(n, pos), gi.DeviceArray(n, vel), gi.DeviceArray(n, np.zeros((n, 4), np.float32))
+        for _ in range(2):
+            gi.force_collision(P, V, F, n, fc)
+            gi.move_collision(P, V, F, n, mc)
+        nb = 64 << 20
+        a, b = C.c_void_p(), C.c_void_p()
+        gi.call("gi_malloc", C.byref(a), nb)
+        gi.call("gi_malloc", C.byref(b), nb)
+        for _ in range(2):
+            gi.call("gi_copy_d2d", b, a, nb, None)
+        gi.sync()
+        log("profile probe done")
         return 0
 
     nt, ni = args.n_target, args.n_impactor
