@@ -918,6 +918,55 @@ int gi_impact_steps(gi_float4* pos, gi_float4* vel, gi_float4* force, int n, con
     return br.end();
 }
 
+int gi_separate_steps(gi_float4* pos, gi_float4* vel, gi_float4* force, int n, const gi_force_separate_constants* fc,
+                      const gi_move_separate_constants* mc, float damp_rate_body1, float damp_rate_body2, int n_steps,
+                      gi_stream_t stream) {
+    int rc = check_particles("gi_separate_steps", pos, vel, force, n);
+    if (rc) return rc;
+    if (!fc || !mc || n_steps < 0)
+        return set_error(GI_E_INVALID_ARG, "gi_separate_steps: invalid arguments");
+    if (!(0 <= fc->boarder1 && fc->boarder1 <= fc->boarder2 && fc->boarder2 <= fc->boarder3 && fc->boarder3 <= n))
+        return set_error(GI_E_INVALID_ARG, "gi_separate_steps: borders must satisfy 0<=b1<=b2<=b3<=n");
+    if (n == 0 || n_steps == 0) return 0;
+    DevCtx* ctx;
+    if ((rc = current_ctx(&ctx))) return rc;
+    StreamBridge br;
+    if ((rc = br.begin(ctx, as_stream(stream)))) return rc;
+    const PairLaw L = law_of(fc);
+    std::vector<SimDesc> sims;
+    sims.push_back(make_sim(0, fc->boarder1, fc->boarder2, L));
+    sims.push_back(make_sim(fc->boarder2, fc->boarder3, n, L));
+    if ((rc = ensure_plan(&ctx->ws, sims, br.run))) return rc;
+    GI_HIP(hipStreamSynchronize(br.run));
+    float4 *P = (float4*)pos, *V = (float4*)vel, *F = (float4*)force;
+    const gi_move_separate_constants m = *mc;
+    std::vector<unsigned char> key;
+    key_append(key, 4);
+    key_append(key, P);
+    key_append(key, V);
+    key_append(key, F);
+    key_append(key, n);
+    key_append(key, *fc);
+    key_append(key, m);
+    key_append(key, damp_rate_body1);
+    key_append(key, damp_rate_body2);
+    hipGraphExec_t step;
+    rc = get_graph(ctx, ctx->graphs, key, ctx->ws.generation,
+                   [&](hipStream_t cs) -> int {
+                       int r = run_force(ctx, &ctx->ws, P, V, F, cs, false);
+                       if (r) return r;
+                       hipLaunchKernelGGL(k_move_damped_separate, dim3(nblk(n, 256)), dim3(256), 0, cs, P, V,
+                                          (const float4*)F, n, m.Dt, m.DtOverMassFe, m.DtOverMassSi, m.boarder1,
+                                          m.boarder2, m.boarder3, damp_rate_body1, damp_rate_body2);
+                       GI_HIP(hipGetLastError());
+                       return 0;
+                   },
+                   &step);
+    if (rc) return rc;
+    for (int s = 0; s < n_steps; ++s) GI_HIP(hipGraphLaunch(step, br.run));
+    return br.end();
+}
+
 // ---- ensemble ---------------------------------------------------------------
 
 int gi_ensemble_create(gi_ensemble** out, int n_sims, const int* n, const gi_force_collision_constants* fc,

@@ -260,6 +260,14 @@ def load() -> C.CDLL:
         "gi_set_tiling": (i, [i, i, C.c_longlong, C.c_longlong, C.c_longlong]),
         "gi_set_batched_tiling": (i, [i, C.c_longlong]),
         "gi_serve_run": (i, [vp, i, vp, vp, vp]),
+        "gi_spin_body": (i, [vp, vp, i, i, i, C.c_double, C.c_double, C.c_double, vp]),
+        "gi_zero_out_drift": (i, [vp, vp, vp]),
+        "gi_make_raw_bodies": (i, [vp, C.c_ulonglong, C.c_double, vp, vp]),
+        "gi_generate_bodies": (i, [vp, C.c_ulonglong, C.c_double, vp, vp, vp, vp, C.c_char_p]),
+        "gi_impact_initialize": (i, [C.c_char_p, C.c_char_p, C.c_char_p]),
+        "gi_impact_run": (i, [C.c_char_p, C.c_char_p, C.c_char_p, vp]),
+        "gi_separate_steps": (i, [vp, vp, vp, i, P(ForceSeparateConstants), P(MoveSeparateConstants), C.c_float,
+                                  C.c_float, i, vp]),
     }
     for name, (res, args) in _optional.items():
         if hasattr(lib, name):
@@ -628,3 +636,65 @@ def serve(requests: list, *, max_particles: int = 0, max_sims: int = 0, quantum_
     call("gi_serve_run", reqs, k, C.byref(opt), res, C.byref(summ))
     out = ([_struct_dict(x) for x in res], _struct_dict(summ))
     return out + (finals,) if keep_final else out
+
+
+# ---------------------------------------------------------------------------
+# Host drivers (generation, impact set-up, impact run)
+# ---------------------------------------------------------------------------
+
+
+class GenerateInfo(C.Structure):
+    _fields_ = [
+        ("n", C.c_int),
+        ("damp_steps", C.c_longlong),
+        ("rest_steps", C.c_longlong),
+        ("spin_rest_steps", C.c_longlong),
+        ("run_time", C.c_double),
+    ]
+
+
+class ImpactRunInfo(C.Structure):
+    _fields_ = [
+        ("n", C.c_int),
+        ("steps", C.c_longlong),
+        ("frames_written", C.c_int),
+        ("start_time", C.c_double),
+        ("end_time", C.c_double),
+    ]
+
+
+def generate_bodies(setup: GeneratorSetup, seed: int = 1, jitter: float = 0.1, out_dir=None):
+    """Run the generation phases; returns (pos, vel, units, info) in the [Fe1, Si1, Fe2, Si2] layout."""
+    n = setup.total_number_of_elements
+    pos = np.zeros((n, 4), np.float32)
+    vel = np.zeros((n, 4), np.float32)
+    u, info = Units(), GenerateInfo()
+    call("gi_generate_bodies", C.byref(setup), seed, jitter, pos.ctypes.data_as(C.c_void_p),
+         vel.ctypes.data_as(C.c_void_p), C.byref(u), C.byref(info),
+         None if out_dir is None else str(out_dir).encode())
+    return pos, vel, u, info
+
+
+def impact_initialize(in_dir, setup_path, out_dir) -> None:
+    call("gi_impact_initialize", str(in_dir).encode(), str(setup_path).encode(), str(out_dir).encode())
+
+
+def impact_run(param_dir, setup_path, run_dir) -> ImpactRunInfo:
+    info = ImpactRunInfo()
+    call("gi_impact_run", str(param_dir).encode(), str(setup_path).encode(), str(run_dir).encode(), C.byref(info))
+    return info
+
+
+def read_frames(path, n: int):
+    """Frames of an impactPosVel file: list of (time, pos (n,4), vel (n,4))."""
+    raw = Path(path).read_bytes()
+    fb = 4 + 32 * n
+    if len(raw) % fb:
+        raise ValueError(f"{path}: size {len(raw)} is not a multiple of the frame size {fb}")
+    out = []
+    for k in range(len(raw) // fb):
+        b = raw[k * fb : (k + 1) * fb]
+        t = np.frombuffer(b[:4], np.float32)[0]
+        a = np.frombuffer(b[4:], np.float32).reshape(2, n, 4)
+        out.append((float(t), a[0].copy(), a[1].copy()))
+    return out

@@ -1,0 +1,441 @@
+// gi_drivers.cpp -- the reference's three compute programs without their
+// display: body generation (generateTargetImpactor.exe), impact set-up
+// (impactInitialization.exe) and the impact run (impactAddTime.exe), on the
+// device-resident steppers of this library and the reference's file formats.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/giant_impact_host.h"
+#include "gi_host_internal.h"
+
+using namespace gi_host;
+
+namespace {
+
+struct DeviceState {
+    void *pos = nullptr, *vel = nullptr, *force = nullptr;
+    int alloc(int n) {
+        const size_t b = (size_t)(n > 0 ? n : 1) * 16;
+        int rc;
+        if ((rc = gi_malloc(&pos, b)) || (rc = gi_malloc(&vel, b)) || (rc = gi_malloc(&force, b))) return rc;
+        return 0;
+    }
+    ~DeviceState() {
+        if (pos) gi_free(pos);
+        if (vel) gi_free(vel);
+        if (force) gi_free(force);
+    }
+};
+
+std::string join(const char *dir, const char *name) {
+    std::string p(dir ? dir : ".");
+    if (!p.empty() && p.back() != '/') p += '/';
+    return p + name;
+}
+
+// Mass-weighted mean of v over a body (iron [a, b), silicate [b, c)) divided
+// by the body's nominal mass, as getCenterOfMass / getAverageLinearVelocity
+// do (generateTargetImpactorBasicFunctions.h:29-80,119-171).
+void body_mean(const gi_float4 *v, int a, int b, int c, double mass_fe, double mass_si, double total,
+               double out[3]) {
+    out[0] = out[1] = out[2] = 0.0;
+    if (total < 0.00000000001) return;
+    for (int i = a; i < b; ++i) {
+        out[0] += v[i].x * mass_fe;
+        out[1] += v[i].y * mass_fe;
+        out[2] += v[i].z * mass_fe;
+    }
+    for (int i = b; i < c; ++i) {
+        out[0] += v[i].x * mass_si;
+        out[1] += v[i].y * mass_si;
+        out[2] += v[i].z * mass_si;
+    }
+    out[0] /= total;
+    out[1] /= total;
+    out[2] /= total;
+}
+
+int write_pos_vel(FILE *f, const gi_float4 *pos, const gi_float4 *vel, int n) {
+    if (std::fwrite(pos, sizeof(gi_float4), n, f) != (size_t)n) return GI_E_IO;
+    if (std::fwrite(vel, sizeof(gi_float4), n, f) != (size_t)n) return GI_E_IO;
+    return 0;
+}
+
+int write_centering(const std::string &path, const float c[6]) {
+    FILE *f = std::fopen(path.c_str(), "wb");
+    if (!f) return GI_E_IO;
+    // Field names and format of impactInitialization.cu:523-531.
+    static const char *names[6] = {"InitialCenterOfMassX",   "InitialCenterOfMassY",   "InitialCenterOfMassZ",
+                                   "InitialLinearVelocityX", "InitialLinearVelocityY", "InitialLinearVelocityZ"};
+    for (int k = 0; k < 6; ++k) std::fprintf(f, "\n %s = %f\n", names[k], c[k]);
+    return std::fclose(f) == 0 ? 0 : GI_E_IO;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gi_spin_body(const gi_float4 *pos, gi_float4 *vel, int start, int stop, int stop_fe, double mass_fe,
+                 double mass_si, double total_mass, const float *spin4) {
+    if (!pos || !vel || !spin4 || start < 0 || stop < start || stop_fe < start || stop_fe > stop)
+        return GI_E_INVALID_ARG;
+    float sx = spin4[0], sy = spin4[1], sz = spin4[2];
+    const float w = spin4[3];
+    // The direction is normalised with a float norm (its operands are float).
+    const double mag = std::sqrt(sx * sx + sy * sy + sz * sz);
+    if (!(0.0000001 < mag)) return GI_E_INVALID_ARG;
+    sx = (float)(sx / mag);
+    sy = (float)(sy / mag);
+    sz = (float)(sz / mag);
+    double com[3];
+    body_mean(pos, start, stop_fe, stop, mass_fe, mass_si, total_mass, com);
+    for (int i = start; i < stop; ++i) {
+        const double rx = pos[i].x - com[0], ry = pos[i].y - com[1], rz = pos[i].z - com[2];
+        // distance from the spin axis, speed 2 pi w d along n = s x r / |s x r|
+        const double r2 = rx * rx + ry * ry + rz * rz;
+        const double along = sx * rx + sy * ry + sz * rz;
+        const double d = std::sqrt(r2 - along * along);
+        const double speed = w * 2.0 * kPi * d;
+        double nx = (sy * rz - sz * ry);
+        double ny = -(sx * rz - sz * rx);
+        double nz = (sx * ry - sy * rx);
+        const double nm = std::sqrt(nx * nx + ny * ny + nz * nz);
+        if (nm != 0.0) {
+            nx /= nm;
+            ny /= nm;
+            nz /= nm;
+            vel[i].x = (float)(vel[i].x + speed * nx);
+            vel[i].y = (float)(vel[i].y + speed * ny);
+            vel[i].z = (float)(vel[i].z + speed * nz);
+        }
+    }
+    return 0;
+}
+
+int gi_zero_out_drift(gi_float4 *pos, gi_float4 *vel, const gi_units *u) {
+    if (!pos || !vel || !u) return GI_E_INVALID_ARG;
+    const int b1 = u->nfe1, b2 = u->nfe1 + u->nsi1, b3 = b2 + u->nfe2, n = b3 + u->nsi2;
+    const int ranges[2][3] = {{0, b1, b2}, {b2, b3, n}};
+    const double mass[2] = {u->mass_of_body1, u->mass_of_body2};
+    for (int b = 0; b < 2; ++b) {
+        double com[3], v[3];
+        body_mean(pos, ranges[b][0], ranges[b][1], ranges[b][2], u->mass_fe, u->mass_si, mass[b], com);
+        body_mean(vel, ranges[b][0], ranges[b][1], ranges[b][2], u->mass_fe, u->mass_si, mass[b], v);
+        for (int i = ranges[b][0]; i < ranges[b][2]; ++i) {
+            pos[i].x = (float)(pos[i].x - com[0]);
+            pos[i].y = (float)(pos[i].y - com[1]);
+            pos[i].z = (float)(pos[i].z - com[2]);
+            vel[i].x = (float)(vel[i].x - v[0]);
+            vel[i].y = (float)(vel[i].y - v[1]);
+            vel[i].z = (float)(vel[i].z - v[2]);
+        }
+    }
+    return 0;
+}
+
+int gi_make_raw_bodies(const gi_units *u, unsigned long long seed, double jitter, gi_float4 *pos, gi_float4 *vel) {
+    if (!u || !pos || !vel) return GI_E_INVALID_ARG;
+    const int n1 = u->nfe1 + u->nsi1, n2 = u->nfe2 + u->nsi2;
+    if (n1 <= 0 || n2 < 0) return GI_E_INVALID_ARG;
+    // The reference draws the raw bodies with srand(time); here a seeded
+    // jittered lattice with an iron core, both bodies centred at the origin,
+    // in the generation layout [Fe1, Si1, Fe2, Si2] with its tags and masses
+    // (generateTargetImpactorBasicFunctions.h:283-398).
+    SplitMix64 rng{seed};
+    const Body bodies[2] = {make_body_count(n1, u->nfe1, jitter, rng), make_body_count(n2, u->nfe2, jitter, rng)};
+    int k = 0;
+    for (int b = 0; b < 2; ++b) {
+        const Body &bd = bodies[b];
+        const int nb = b == 0 ? n1 : n2;
+        for (int q = 0; q < nb; ++q, ++k) {
+            const bool fe = q < bd.nfe;
+            pos[k] = gi_float4{(float)bd.x[q], (float)bd.y[q], (float)bd.z[q], (float)(2 * b + (fe ? 0 : 1))};
+            vel[k] = gi_float4{0.0f, 0.0f, 0.0f, (float)(fe ? u->mass_fe : u->mass_si)};
+        }
+    }
+    return 0;
+}
+
+int gi_generate_bodies(const gi_generator_setup *setup, unsigned long long seed, double jitter, gi_float4 *pos,
+                       gi_float4 *vel, gi_units *units, gi_generate_info *info, const char *out_dir) {
+    if (!setup || !pos || !vel) return GI_E_INVALID_ARG;
+    gi_units u;
+    int rc = gi_units_compute(setup, &u);
+    if (rc) return rc;
+    const int n = setup->total_number_of_elements;
+    const int n1 = u.nfe1 + u.nsi1, n2 = u.nfe2 + u.nsi2;
+    if (n <= 0 || n1 + n2 != n) return GI_E_INVALID_ARG;
+    gi_force_separate_constants fc;
+    gi_move_separate_constants mc;
+    gi_separate_constants(&u, setup->dt, &fc, &mc);
+    if ((rc = gi_make_raw_bodies(&u, seed, jitter, pos, vel))) return rc;
+
+    // Phase lengths with the reference's bookkeeping: after step k the time is
+    // the double sum of k Dt's; the step moves with damping while that time is
+    // below DampTime, rests until DampTime + DampRestTime, spins once, rests,
+    // and the run ends after the step whose time exceeds the total
+    // (generateTargetImpactor.cu:671-727).
+    long long n_damp = 0, n_rest = 0, n_spin = 0;
+    double t = 0.0;
+    const double t_end = u.damp_time + u.damp_rest_time + u.spin_rest_time;
+    for (;;) {
+        t += setup->dt;
+        if (t < u.damp_time)
+            ++n_damp;
+        else if (t < u.damp_time + u.damp_rest_time)
+            ++n_rest;
+        else
+            ++n_spin;
+        if (t_end < t) break;
+        if (n_damp + n_rest + n_spin > (1LL << 31)) return GI_E_TOO_LARGE;
+    }
+
+    DeviceState d;
+    if ((rc = d.alloc(n))) return rc;
+    const size_t bytes = (size_t)n * 16;
+    if ((rc = gi_copy_h2d(d.pos, pos, bytes)) || (rc = gi_copy_h2d(d.vel, vel, bytes))) return rc;
+    gi_float4 *P = (gi_float4 *)d.pos, *V = (gi_float4 *)d.vel, *F = (gi_float4 *)d.force;
+    if ((rc = gi_separate_steps(P, V, F, n, &fc, &mc, setup->damp_rate_body1, setup->damp_rate_body2, (int)n_damp,
+                                nullptr)))
+        return rc;
+    if ((rc = gi_separate_steps(P, V, F, n, &fc, &mc, 0.0f, 0.0f, (int)n_rest, nullptr))) return rc;
+    if (n_spin > 0) {
+        // The spin step: force on the unspun state, then spin both bodies
+        // (on the host, as spinBody), then move (generateTargetImpactor.cu:698-711).
+        if ((rc = gi_force_separate(P, V, F, n, &fc, nullptr))) return rc;
+        if ((rc = gi_copy_d2h(pos, P, bytes)) || (rc = gi_copy_d2h(vel, V, bytes))) return rc;
+        const float s1[4] = {setup->initial_spin1[0], setup->initial_spin1[1], setup->initial_spin1[2], u.spin1_w};
+        const float s2[4] = {setup->initial_spin2[0], setup->initial_spin2[1], setup->initial_spin2[2], u.spin2_w};
+        if ((rc = gi_spin_body(pos, vel, 0, n1, u.nfe1, u.mass_fe, u.mass_si, u.mass_of_body1, s1))) return rc;
+        if ((rc = gi_spin_body(pos, vel, n1, n, n1 + u.nfe2, u.mass_fe, u.mass_si, u.mass_of_body2, s2))) return rc;
+        if ((rc = gi_copy_h2d(V, vel, bytes))) return rc;
+        if ((rc = gi_move_damped_separate(P, V, F, n, &mc, 0.0f, 0.0f, nullptr))) return rc;
+        if ((rc = gi_separate_steps(P, V, F, n, &fc, &mc, 0.0f, 0.0f, (int)(n_spin - 1), nullptr))) return rc;
+    }
+    if ((rc = gi_device_sync())) return rc;
+    if ((rc = gi_copy_d2h(pos, P, bytes)) || (rc = gi_copy_d2h(vel, V, bytes))) return rc;
+    if ((rc = gi_zero_out_drift(pos, vel, &u))) return rc;
+
+    if (out_dir) {
+        // recordCarryForwardParameters / recordTargetImpactorInitialPosVel
+        // (generateTargetImpactorBasicFunctions.h:688-722).
+        if ((rc = gi_parameters_write(join(out_dir, "targetImpactorParameters").c_str(), &u, n))) return rc;
+        FILE *f = std::fopen(join(out_dir, "targetImpactorInitialPosVel").c_str(), "wb");
+        if (!f) return GI_E_IO;
+        rc = std::fwrite(&n, sizeof(int), 1, f) == 1 ? write_pos_vel(f, pos, vel, n) : GI_E_IO;
+        if (std::fclose(f) != 0 && !rc) rc = GI_E_IO;
+        if (rc) return rc;
+    }
+    if (units) *units = u;
+    if (info) {
+        info->n = n;
+        info->damp_steps = n_damp;
+        info->rest_steps = n_rest;
+        info->spin_rest_steps = n_spin;
+        info->run_time = t;
+    }
+    return 0;
+}
+
+int gi_impact_initialize(const char *in_dir, const char *setup_path, const char *out_dir) {
+    if (!in_dir || !setup_path || !out_dir) return GI_E_INVALID_ARG;
+    gi_units u;
+    int n = 0;
+    int rc = gi_parameters_read(join(in_dir, "targetImpactorParameters").c_str(), &u, &n);
+    if (rc) return rc;
+    if (n <= 0 || u.nfe1 + u.nsi1 + u.nfe2 + u.nsi2 != n) return GI_E_PARSE;
+    // Body offsets (km) and velocities (km/s), positional
+    // (impactInitialization.cu:188-217), stored as float like the reference's
+    // float4 globals and converted to simulation units (:235-249).
+    float ip1[3], ip2[3], iv1[3], iv2[3];
+    {
+        PositionalReader r(setup_path);
+        if (!r.ok()) return GI_E_IO;
+        bool ok = true;
+        for (float *a : {ip1, ip2, iv1, iv2})
+            for (int k = 0; k < 3; ++k) ok = ok && r.next(a[k]);
+        if (!ok) return GI_E_PARSE;
+    }
+    for (int k = 0; k < 3; ++k) {
+        ip1[k] = (float)(ip1[k] / u.unit_length);
+        ip2[k] = (float)(ip2[k] / u.unit_length);
+        iv1[k] = (float)(iv1[k] * (u.unit_time / u.unit_length));
+        iv2[k] = (float)(iv2[k] * (u.unit_time / u.unit_length));
+    }
+    std::vector<gi_float4> tp(n), tv(n), pos(n), vel(n);
+    {
+        FILE *f = std::fopen(join(in_dir, "targetImpactorInitialPosVel").c_str(), "rb");
+        if (!f) return GI_E_IO;
+        int lead = 0;
+        const bool ok = std::fread(&lead, sizeof(int), 1, f) == 1 &&
+                        std::fread(tp.data(), sizeof(gi_float4), n, f) == (size_t)n &&
+                        std::fread(tv.data(), sizeof(gi_float4), n, f) == (size_t)n;
+        std::fclose(f);
+        if (!ok) return GI_E_IO;
+    }
+    const int n1 = u.nfe1 + u.nsi1;
+    for (int i = 0; i < n; ++i) {
+        const float *dp = i < n1 ? ip1 : ip2;
+        const float *dv = i < n1 ? iv1 : iv2;
+        tp[i].x += dp[0];
+        tp[i].y += dp[1];
+        tp[i].z += dp[2];
+        tv[i].x += dv[0];
+        tv[i].y += dv[1];
+        tv[i].z += dv[2];
+    }
+    // [Fe1, Si1, Fe2, Si2] -> [Fe1, Fe2, Si1, Si2] (impactInitialization.cu:329-371).
+    const int seg[4][2] = {{0, u.nfe1}, {n1, n1 + u.nfe2}, {u.nfe1, n1}, {n1 + u.nfe2, n}};
+    int k = 0;
+    for (auto &s : seg)
+        for (int i = s[0]; i < s[1]; ++i, ++k) {
+            pos[k] = tp[i];
+            vel[k] = tv[i];
+        }
+    {
+        FILE *f = std::fopen(join(out_dir, "impactPosVel").c_str(), "wb");
+        if (!f) return GI_E_IO;
+        const float time0 = 0.0f;
+        rc = std::fwrite(&time0, sizeof(float), 1, f) == 1 ? write_pos_vel(f, pos.data(), vel.data(), n) : GI_E_IO;
+        if (std::fclose(f) != 0 && !rc) rc = GI_E_IO;
+        if (rc) return rc;
+    }
+    // Centre of mass and mean velocity of the whole system in double
+    // (impactInitialization.cu:395-446), kept as float for the recentring.
+    const int nfe = u.nfe1 + u.nfe2;
+    double mt = 0.0, c[6] = {0, 0, 0, 0, 0, 0};
+    for (int i = 0; i < n; ++i) mt += i < nfe ? u.mass_fe : u.mass_si;
+    for (int i = 0; i < n; ++i) {
+        const double m = i < nfe ? u.mass_fe : u.mass_si;
+        c[0] += pos[i].x * m;
+        c[1] += pos[i].y * m;
+        c[2] += pos[i].z * m;
+    }
+    for (int i = 0; i < n; ++i) {
+        const double m = i < nfe ? u.mass_fe : u.mass_si;
+        c[3] += vel[i].x * m;
+        c[4] += vel[i].y * m;
+        c[5] += vel[i].z * m;
+    }
+    float cf[6];
+    for (int q = 0; q < 6; ++q) cf[q] = (float)(c[q] / mt);
+    return write_centering(join(out_dir, "viewCenteringInfo"), cf);
+}
+
+int gi_impact_run(const char *param_dir, const char *setup_path, const char *run_dir, gi_impact_run_info *info) {
+    if (!param_dir || !setup_path || !run_dir) return GI_E_INVALID_ARG;
+    gi_units u;
+    int n = 0;
+    int rc = gi_parameters_read(join(param_dir, "targetImpactorParameters").c_str(), &u, &n);
+    if (rc) return rc;
+    if (n <= 0 || u.nfe1 + u.nsi1 + u.nfe2 + u.nsi2 != n) return GI_E_PARSE;
+    // setupImpactAddTime, positional (impactAddTime.cu:60-125).
+    double additional_time = 0.0, dt = 0.0;
+    int draw_rate = 0, record_rate = 0, recenter_rate = 0;
+    {
+        PositionalReader r(setup_path);
+        if (!r.ok()) return GI_E_IO;
+        if (!(r.next(additional_time) && r.next(dt) && r.next(draw_rate) && r.next(record_rate) &&
+              r.next(recenter_rate)))
+            return GI_E_PARSE;
+    }
+    if (dt <= 0.0 || record_rate <= 0 || recenter_rate <= 0) return GI_E_INVALID_ARG;
+    additional_time *= 24.0 * 60.0 * 60.0;
+    additional_time /= u.unit_time;
+    float centering[6];
+    {
+        PositionalReader r(join(run_dir, "viewCenteringInfo").c_str());
+        if (!r.ok()) return GI_E_IO;
+        for (float &v : centering)
+            if (!r.next(v)) return GI_E_PARSE;
+    }
+    gi_force_collision_constants fc;
+    gi_move_collision_constants mc;
+    gi_impact_constants(&u, dt, &fc, &mc);
+
+    // Resume from the last frame (impactAddTime.cu:205-243).
+    std::vector<gi_float4> pos(n), vel(n);
+    const std::string frames = join(run_dir, "impactPosVel");
+    FILE *ff = std::fopen(frames.c_str(), "rb+");
+    if (!ff) return GI_E_IO;
+    const long frame_bytes = (long)(sizeof(float) + 2 * (size_t)n * sizeof(gi_float4));
+    float time0 = 0.0f;
+    if (std::fseek(ff, -frame_bytes, SEEK_END) != 0 || std::fread(&time0, sizeof(float), 1, ff) != 1 ||
+        std::fread(pos.data(), sizeof(gi_float4), n, ff) != (size_t)n ||
+        std::fread(vel.data(), sizeof(gi_float4), n, ff) != (size_t)n) {
+        std::fclose(ff);
+        return GI_E_IO;
+    }
+    double run_time = (double)time0;
+    const double total_run_time = run_time + additional_time;
+
+    // Step count: the loop ends after the first step whose time exceeds the
+    // total (impactAddTime.cu:405,457).
+    long long n_steps = 0;
+    {
+        double t = run_time;
+        do {
+            t += dt;
+            ++n_steps;
+        } while (!(total_run_time < t));
+    }
+
+    DeviceState d;
+    if ((rc = d.alloc(n))) {
+        std::fclose(ff);
+        return rc;
+    }
+    const size_t bytes = (size_t)n * 16;
+    gi_float4 *P = (gi_float4 *)d.pos, *V = (gi_float4 *)d.vel, *F = (gi_float4 *)d.force;
+    if ((rc = gi_copy_h2d(P, pos.data(), bytes)) || (rc = gi_copy_h2d(V, vel.data(), bytes))) {
+        std::fclose(ff);
+        return rc;
+    }
+    const double tc[3] = {centering[0], centering[1], centering[2]};
+    const double tv[3] = {centering[3], centering[4], centering[5]};
+    const int nfe = u.nfe1 + u.nfe2;
+    int frames_written = 0;
+    long long done = 0;
+    const double start_time = run_time;
+    while (done < n_steps) {
+        // run to the next record / recentre event or the end
+        const long long next_record = (done / record_rate + 1) * record_rate;
+        const long long next_recenter = (done / recenter_rate + 1) * recenter_rate;
+        const long long target = std::min(std::min(next_record, next_recenter), n_steps);
+        const int k = (int)(target - done);
+        if ((rc = gi_impact_steps(P, V, F, n, &fc, &mc, k, 0, nullptr, nullptr, nullptr))) break;
+        for (int q = 0; q < k; ++q) run_time += dt;
+        done = target;
+        if (done % record_rate == 0) {
+            if ((rc = gi_device_sync())) break;
+            if ((rc = gi_copy_d2h(pos.data(), P, bytes)) || (rc = gi_copy_d2h(vel.data(), V, bytes))) break;
+            const float tf = (float)run_time;
+            if (std::fseek(ff, 0, SEEK_END) != 0 || std::fwrite(&tf, sizeof(float), 1, ff) != 1 ||
+                write_pos_vel(ff, pos.data(), vel.data(), n) != 0) {
+                rc = GI_E_IO;
+                break;
+            }
+            ++frames_written;
+        }
+        if (done % recenter_rate == 0) {
+            if ((rc = gi_recenter(P, V, n, nfe, u.mass_fe, u.mass_si, tc, tv, nullptr))) break;
+        }
+    }
+    if (!rc) rc = gi_device_sync();
+    if (std::fclose(ff) != 0 && !rc) rc = GI_E_IO;
+    if (rc) return rc;
+    if ((rc = write_centering(join(run_dir, "viewCenteringInfo"), centering))) return rc;
+    if (info) {
+        info->n = n;
+        info->steps = n_steps;
+        info->frames_written = frames_written;
+        info->start_time = start_time;
+        info->end_time = run_time;
+    }
+    return 0;
+}
+
+}  // extern "C"

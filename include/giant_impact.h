@@ -212,6 +212,13 @@ int gi_impact_steps(gi_float4 *pos, gi_float4 *vel, gi_float4 *force, int n,
                     int n_steps, int recenter_rate, const double *target_com3,
                     const double *target_vel3, gi_stream_t stream);
 
+/* The generation phases' step (generateTargetImpactor.cu:670,684,696,711):
+ * n_steps of (gi_force_separate, gi_move_damped_separate with damp1/damp2),
+ * captured once in a HIP graph and replayed. */
+int gi_separate_steps(gi_float4 *pos, gi_float4 *vel, gi_float4 *force, int n,
+                      const gi_force_separate_constants *fc, const gi_move_separate_constants *mc,
+                      float damp_rate_body1, float damp_rate_body2, int n_steps, gi_stream_t stream);
+
 /* ---- batched ensemble (many impacts of uneven n in one launch per kernel) - */
 
 typedef struct gi_ensemble gi_ensemble;

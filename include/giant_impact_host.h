@@ -97,6 +97,70 @@ int gi_make_impact(const gi_impact_config *cfg, gi_float4 *pos, gi_float4 *vel,
                    gi_force_collision_constants *fc, gi_move_collision_constants *mc,
                    gi_impact_info *info);
 
+/* ---- generation phase (generateTargetImpactor.cu:666-749) ---------------- */
+
+/* spinBody (generateTargetImpactorBasicFunctions.h:407-470): add the rigid
+ * spin of spin4 = {direction xyz, revolutions per time unit} about the
+ * body's centre of mass to vel[start, stop); iron is [start, stop_fe).  The
+ * centre of mass divides by total_mass (the body's nominal mass, as the
+ * reference does).  Host arrays.  GI_E_INVALID_ARG for a zero direction. */
+int gi_spin_body(const gi_float4 *pos, gi_float4 *vel, int start, int stop, int stop_fe,
+                 double mass_fe, double mass_si, double total_mass, const float *spin4);
+/* zeroOutTargetImpactorDrift (generateTargetImpactorBasicFunctions.h:472-506):
+ * move each body's centre of mass to the origin and remove its mean velocity.
+ * Generation layout [Fe1, Si1, Fe2, Si2] with counts from u.  Host arrays. */
+int gi_zero_out_drift(gi_float4 *pos, gi_float4 *vel, const gi_units *u);
+
+/* The raw bodies the generation starts from: seeded jittered lattices (the
+ * reference's createRawTargetImpactor uses srand(time)), iron innermost,
+ * both centred at the origin, layout [Fe1, Si1, Fe2, Si2], pos.w = tag 0..3,
+ * vel.w = element mass, zero velocity.  Host arrays of n = sum of u's counts. */
+int gi_make_raw_bodies(const gi_units *u, unsigned long long seed, double jitter, gi_float4 *pos,
+                       gi_float4 *vel);
+
+typedef struct gi_generate_info {
+    int n;
+    long long damp_steps, rest_steps, spin_rest_steps; /* steps per phase */
+    double run_time;                                    /* simulation time at the end */
+} gi_generate_info;
+
+/* The generation run of generateTargetImpactor.exe without its display:
+ * seeded raw bodies (jittered lattices, iron cores, layout [Fe1, Si1, Fe2,
+ * Si2], both centred at the origin), then damped settling for DampTime, a
+ * rest for DampRestTime, the spin of both bodies, a rest for SpinRestTime,
+ * and the drift removal, with the reference's step bookkeeping.  pos/vel are
+ * host arrays of setup->total_number_of_elements records (the result).  When
+ * out_dir is not NULL the reference's files targetImpactorParameters and
+ * targetImpactorInitialPosVel are written there. */
+int gi_generate_bodies(const gi_generator_setup *setup, unsigned long long seed, double jitter,
+                       gi_float4 *pos, gi_float4 *vel, gi_units *units, gi_generate_info *info,
+                       const char *out_dir);
+
+/* ---- impact initialisation and the impact run (impactInitialization.cu,
+ *      impactAddTime.cu) ---------------------------------------------------- */
+
+/* setupInitialConditions + recordInitialImpactStat's centring record
+ * (impactInitialization.cu:260-391,393-533): read targetImpactorInitialPosVel
+ * and targetImpactorParameters from in_dir and setupImpactInitialization from
+ * setup_path; place and push the bodies; reorder to [Fe1, Fe2, Si1, Si2];
+ * write frame 0 to out_dir/impactPosVel and out_dir/viewCenteringInfo. */
+int gi_impact_initialize(const char *in_dir, const char *setup_path, const char *out_dir);
+
+typedef struct gi_impact_run_info {
+    int n;
+    long long steps;
+    int frames_written;
+    double start_time, end_time; /* simulation units */
+} gi_impact_run_info;
+
+/* impactAddTime.exe without its display: read targetImpactorParameters from
+ * param_dir, setupImpactAddTime from setup_path, viewCenteringInfo and the
+ * last frame of impactPosVel from run_dir; advance AdditionalTime days,
+ * appending a frame every RecordRate steps and recentring every RecenterRate
+ * steps (FP64 on the device); rewrite viewCenteringInfo at the end. */
+int gi_impact_run(const char *param_dir, const char *setup_path, const char *run_dir,
+                  gi_impact_run_info *info);
+
 /* ---- request loop (continuous batching of impact-sweep jobs) ------------- */
 
 typedef struct gi_serve_request {

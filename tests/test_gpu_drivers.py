@@ -1,0 +1,133 @@
+"""The headless drivers: generation phases (damp, rest, spin, rest, drift
+removal), impact set-up and the impact run with the reference's file formats
+(frames, resume, recentring)."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from conftest import f4
+
+pytestmark = pytest.mark.gpu
+
+
+def _short_setup(gi, n=1024, hours=(0.1, 0.1, 0.1)):
+    s = gi.generator_setup_defaults(n)
+    s.damp_time, s.damp_rest_time, s.spin_rest_time = hours
+    return s
+
+
+def _phase_counts(u, dt):
+    t, nd, nr, ns = 0.0, 0, 0, 0
+    end = u.damp_time + u.damp_rest_time + u.spin_rest_time
+    while True:
+        t += dt
+        if t < u.damp_time:
+            nd += 1
+        elif t < u.damp_time + u.damp_rest_time:
+            nr += 1
+        else:
+            ns += 1
+        if end < t:
+            return nd, nr, ns, t
+
+
+def test_generation_phases_match_the_oracle(gpu, oracle):
+    s = _short_setup(gpu)
+    pos, vel, u, info = gpu.generate_bodies(s, seed=5)
+    nd, nr, ns, t = _phase_counts(u, s.dt)
+    assert (info.damp_steps, info.rest_steps, info.spin_rest_steps) == (nd, nr, ns)
+    assert info.run_time == t
+    # the same phases on the CPU oracle from the same raw bodies
+    n = info.n
+    p0, v0 = f4(n), f4(n)
+    gpu.load().gi_make_raw_bodies(C.byref(u), 5, 0.1, p0.ctypes.data, v0.ctypes.data)
+    fs, ms = gpu.separate_constants(u, s.dt)
+    p, v = p0, v0
+    n1 = u.nfe1 + u.nsi1
+    for k in range(nd + nr + ns):
+        F = oracle.force_separate_f32(p, v, fs)
+        if k == nd + nr:
+            v = oracle.spin_body(p, v, 0, n1, u.nfe1, u.mass_fe, u.mass_si, u.mass_of_body1,
+                                 [s.initial_spin1[0], s.initial_spin1[1], s.initial_spin1[2], u.spin1_w])
+            v = oracle.spin_body(p, v, n1, n, n1 + u.nfe2, u.mass_fe, u.mass_si, u.mass_of_body2,
+                                 [s.initial_spin2[0], s.initial_spin2[1], s.initial_spin2[2], u.spin2_w])
+        d = (s.damp_rate_body1, s.damp_rate_body2) if k < nd else (0.0, 0.0)
+        p, v = oracle.move_damped_separate_f32(p, v, F, ms, *d)
+    gpu.load().gi_zero_out_drift(p.ctypes.data, v.ctypes.data, C.byref(u))
+    scale = np.abs(p[:, :3]).max()
+    print("max |dx|", np.abs(pos[:, :3] - p[:, :3]).max(), "scale", scale)
+    assert np.abs(pos[:, :3] - p[:, :3]).max() <= 1e-4 * scale
+    assert np.abs(vel[:, :3] - v[:, :3]).max() <= 1e-3 * np.abs(v[:, :3]).max()
+    # drift removed, tags and masses passed through
+    assert np.array_equal(pos[:, 3], p0[:, 3]) and np.array_equal(vel[:, 3], v0[:, 3])
+
+
+def test_impact_pipeline_files_resume_and_recenter(gpu, tmp_path):
+    s = _short_setup(gpu, n=1024)
+    bodies = tmp_path / "bodies"
+    bodies.mkdir()
+    _, _, u, _ = gpu.generate_bodies(s, seed=6, out_dir=bodies)
+    n = 1024
+    assert (bodies / "targetImpactorInitialPosVel").stat().st_size == 4 + 32 * n
+    # place the bodies 2 body-radii apart, moving toward each other (km, km/s)
+    r_km = 8.0 * u.unit_length
+    setup = tmp_path / "setupImpactInitialization"
+    setup.write_text(
+        f"Body 1 position\nx = {-r_km};\ny = -20.0;\nz = 0.0;\n"
+        f"Body 2 position\nx = {r_km};\ny = 20.0;\nz = 0.0;\n"
+        "Body 1 velocity\nx = 2.0;\ny = 0.0;\nz = 0.0;\n"
+        "Body 2 velocity\nx = -2.0;\ny = 0.0;\nz = 0.0;\n"
+    )
+    run = tmp_path / "run"
+    run.mkdir()
+    gpu.impact_initialize(bodies, setup, run)
+    frames = gpu.read_frames(run / "impactPosVel", n)
+    assert len(frames) == 1 and frames[0][0] == 0.0
+    fc, _ = gpu.impact_constants(u)
+    # frame 0 is in the impact layout [Fe1, Fe2, Si1, Si2]
+    tags = frames[0][1][:, 3]
+    nfe = u.nfe1 + u.nfe2
+    assert np.all(np.isin(tags[:nfe], [0.0, 2.0])) and np.all(np.isin(tags[nfe:], [1.0, 3.0]))
+    days = 600 * 0.002 * u.unit_time / 86400.0  # about 600 steps
+    add = tmp_path / "setupImpactAddTime"
+    add.write_text(f"AdditionTime = {days:.12f};\nDt = 0.002;\nDrawRate = 100;\nRecordRate = 100;\nRecenterRate = 1000000;\n")
+    info1 = gpu.impact_run(bodies, add, run)
+    assert info1.frames_written == info1.steps // 100
+    f1 = gpu.read_frames(run / "impactPosVel", n)
+    assert len(f1) == 1 + info1.frames_written
+    times = [f[0] for f in f1]
+    assert all(b > a for a, b in zip(times, times[1:]))
+    # resume: the second run starts from the last frame
+    info2 = gpu.impact_run(bodies, add, run)
+    assert info2.start_time == pytest.approx(times[-1])
+    f2 = gpu.read_frames(run / "impactPosVel", n)
+    assert len(f2) == len(f1) + info2.frames_written
+    # the same total in one run from frame 0 reproduces both runs' frames bitwise
+    run_b = tmp_path / "run_b"
+    run_b.mkdir()
+    gpu.impact_initialize(bodies, setup, run_b)
+    steps_total = (len(f2) - 1) * 100
+    add_b = tmp_path / "setupImpactAddTime_b"
+    add_b.write_text(f"AdditionTime = {(steps_total - 0.5) * 0.002 * u.unit_time / 86400.0:.12f};\nDt = 0.002;\n"
+                     "DrawRate = 100;\nRecordRate = 100;\nRecenterRate = 1000000;\n")
+    gpu.impact_run(bodies, add_b, run_b)
+    fb = gpu.read_frames(run_b / "impactPosVel", n)
+    assert len(fb) >= len(f2)
+    for k in range(1, len(f2)):
+        assert np.array_equal(fb[k][1], f2[k][1]) and np.array_equal(fb[k][2], f2[k][2]), k
+    # recentring every 100 steps keeps the centre of mass at the recorded target
+    run_c = tmp_path / "run_c"
+    run_c.mkdir()
+    gpu.impact_initialize(bodies, setup, run_c)
+    target = [float(x.split("=")[1]) for x in (run_c / "viewCenteringInfo").read_text().split("\n") if "=" in x]
+    add_c = tmp_path / "setupImpactAddTime_c"
+    add_c.write_text(f"AdditionTime = {days:.12f};\nDt = 0.002;\nDrawRate = 100;\nRecordRate = 50;\nRecenterRate = 100;\n")
+    gpu.impact_run(bodies, add_c, run_c)
+    fcn = gpu.read_frames(run_c / "impactPosVel", n)
+    m = np.where(np.arange(n) < nfe, u.mass_fe, 1.0)
+    t_last, p_last, _ = fcn[-1]
+    com = (m[:, None] * p_last[:, :3]).sum(0) / m.sum()
+    assert np.allclose(com, target[:3], atol=0.05)

@@ -17,7 +17,9 @@ def _requests(k=6, seed=11):
         ni = max(1, n // 4)
         out.append(dict(n_target=n - ni, n_impactor=ni, seed=1000 + i, angle_deg=float(rng.uniform(30, 60)),
                         speed_vesc=float(rng.uniform(1.0, 1.4)), target_spin=float(rng.uniform(0, 0.6)),
-                        steps=int(rng.integers(20, 150)), arrival_s=0.05 * i))
+                        steps=int(rng.integers(20, 150)),
+                        # three arrive together (they must share the device), the rest later
+                        arrival_s=0.0 if i < 3 else 0.05 * (i - 2)))
     return out
 
 

@@ -161,3 +161,41 @@ def test_library_exports_every_declared_symbol(gi):
     lib = gi.load()
     for d in declared:
         assert getattr(lib, d) is not None
+
+
+def test_spin_body_is_bitwise_the_oracle(gi, oracle):
+    u = gi.units(gi.generator_setup_defaults(2048))
+    n = u.nfe1 + u.nsi1 + u.nfe2 + u.nsi2
+    pos, vel = f4(n), f4(n)
+    assert gi.load().gi_make_raw_bodies(C.byref(u), 3, 0.1, pos.ctypes.data, vel.ctypes.data) == 0
+    rng = np.random.default_rng(1)
+    vel[:, :3] = rng.normal(0, 0.1, (n, 3))
+    n1 = u.nfe1 + u.nsi1
+    spin = np.array([0.2, 1.0, -0.3, 0.043], np.float32)
+    v_gi = vel.copy()
+    assert gi.load().gi_spin_body(pos.ctypes.data, v_gi.ctypes.data, 0, n1, u.nfe1, u.mass_fe, u.mass_si,
+                                  u.mass_of_body1, spin.ctypes.data) == 0
+    v_or = oracle.spin_body(pos, vel, 0, n1, u.nfe1, u.mass_fe, u.mass_si, u.mass_of_body1, spin)
+    assert np.array_equal(v_gi, v_or)
+    assert np.array_equal(v_gi[n1:], vel[n1:])  # body 2 untouched
+    zero = np.zeros(4, np.float32)
+    assert gi.load().gi_spin_body(pos.ctypes.data, v_gi.ctypes.data, 0, n1, u.nfe1, u.mass_fe, u.mass_si,
+                                  u.mass_of_body1, zero.ctypes.data) == -1
+
+
+def test_zero_out_drift_uses_the_nominal_body_masses(gi, oracle):
+    u = gi.units(gi.generator_setup_defaults(1024))
+    n = u.nfe1 + u.nsi1 + u.nfe2 + u.nsi2
+    pos, vel = f4(n), f4(n)
+    gi.load().gi_make_raw_bodies(C.byref(u), 4, 0.1, pos.ctypes.data, vel.ctypes.data)
+    rng = np.random.default_rng(2)
+    pos[:, :3] += rng.normal(0, 3.0, (1, 3)).astype(np.float32)
+    vel[:, :3] = rng.normal(0.5, 0.2, (n, 3))
+    p, v = pos.copy(), vel.copy()
+    assert gi.load().gi_zero_out_drift(p.ctypes.data, v.ctypes.data, C.byref(u)) == 0
+    b1, b2, b3 = u.nfe1, u.nfe1 + u.nsi1, u.nfe1 + u.nsi1 + u.nfe2
+    for (a, f, e, M) in ((0, b1, b2, u.mass_of_body1), (b2, b3, n, u.mass_of_body2)):
+        com = oracle.body_com(pos, a, f, e, u.mass_fe, u.mass_si, M)[:3]
+        vm = oracle.body_com(vel, a, f, e, u.mass_fe, u.mass_si, M)[:3]
+        assert np.array_equal(p[a:e, :3], (pos[a:e, :3].astype(np.float64) - com).astype(np.float32))
+        assert np.array_equal(v[a:e, :3], (vel[a:e, :3].astype(np.float64) - vm).astype(np.float32))
