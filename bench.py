@@ -10,9 +10,11 @@ state resident in HBM.
 
 Printed: one JSON line.  value = pair interactions per second of whole steps
 (N^2 per step, self pairs included, as the reference kernel visits them).
-roofline: the all-pairs gravity kernel, 20 FP32 flops per pair against the
-157.3 TFLOP/s FP32 peak, timed with HIP events on its stream during the
-timed steps.  cpu_baseline: the oracle's float restatement of the reference
+roofline: the all-pairs kernel (k_mutual, each pair evaluated once for both
+particles, or k_gravity, one-sided, as the tuning table picks), N^2 pair
+interactions x 20 FP32 flops against the 157.3 TFLOP/s FP32 peak, timed with
+HIP events on its stream during the timed steps; pass_ms adds the padding and
+per-particle sum kernels of the mutual pass.  cpu_baseline: the oracle's float restatement of the reference
 kernel timed on this host's cores over a bounded sample of rows.
 """
 from __future__ import annotations
@@ -227,14 +229,18 @@ def main() -> int:
     L, T = C.c_longlong(), C.c_double()
     gi.call("gi_kernel_timing", b"gravity", C.byref(L), C.byref(T))
     grav_s = T.value / 1e3 / max(L.value, 1)
+    gi.call("gi_kernel_timing", b"gravity_pass", C.byref(L), C.byref(T))
+    pass_s = T.value / 1e3 / max(L.value, 1)
     gi.call("gi_kernel_timing", b"move", C.byref(L), C.byref(T))
     move_s = T.value / 1e3 / max(L.value, 1)
-    log(f"{args.steps} steps in {elapsed:.3f} s; gravity kernel {grav_s * 1e3:.3f} ms; move kernel {move_s * 1e6:.1f} us")
+    plan = gi.plan_info()
+    log(f"{args.steps} steps in {elapsed:.3f} s; {plan['kernel']} {grav_s * 1e3:.3f} ms (pass {pass_s * 1e3:.3f} ms); "
+        f"move kernel {move_s * 1e6:.1f} us; plan {plan}")
 
     pairs = float(n) * float(n)
     value = pairs * args.steps / elapsed
     achieved = pairs * FLOPS_PER_PAIR / grav_s / 1e12
-    tr = traffic_from_profiles("k_gravity")
+    tr = traffic_from_profiles(plan["kernel"])
     traffic = int(tr[1]) if tr and tr[0] == n else None
     result = {
         "metric": "pair-int/s",
@@ -258,7 +264,7 @@ def main() -> int:
         },
         "roofline": {
             "bound": "valu",
-            "kernel": "k_gravity",
+            "kernel": plan["kernel"],
             "achieved": achieved,
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
@@ -266,6 +272,9 @@ def main() -> int:
             "traffic": traffic,
             "pairs_per_s": pairs / grav_s,
             "avg_launch_ms": grav_s * 1e3,
+            "pass_ms": pass_s * 1e3,
+            "frac_pass": pairs * FLOPS_PER_PAIR / pass_s / 1e12 / PEAK_FP32_TFLOPS,
+            "plan": {k: plan[k] for k in ("r", "pair_tasks", "chunk_len")},
         },
         "move_kernel": {
             "avg_launch_us": move_s * 1e6,

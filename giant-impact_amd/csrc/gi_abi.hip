@@ -76,7 +76,9 @@ struct DevBuf {
 };
 
 // HIP events around one timed launch; kind 0 = all-pairs gravity, 1 = move.
-constexpr int kTimedKinds = 2;
+// kind 2 = the whole all-pairs pass (for the mutual form: padding, the
+// all-pairs kernel and the per-particle sums).
+constexpr int kTimedKinds = 3;
 struct TimingPair {
     hipEvent_t a = nullptr, b = nullptr;
     int kind = 0;
@@ -84,19 +86,32 @@ struct TimingPair {
 
 // The plan of one force evaluation over a set of packed simulations.
 struct Plan {
-    std::vector<SimDesc> sims;
+    std::vector<SimDesc> caller_sims;  // the simulations as requested
+    std::vector<SimDesc> sims;         // with the planner's fields filled in
     std::vector<GravTask> tasks;
     std::vector<int> block_sim, block_first, sim_first_block;
     Tiling tiling{4, 0, 0, 0, 0};
     int ntot = 0;
     int partial_len = 0;  // float4 elements
     int buckets = 0;      // contact-grid buckets over all simulations
+    // mutual all-pairs pass (tiling.variant == 2)
+    bool mutual = false;
+    std::vector<MutualTask> mtasks;
+    std::vector<RowDesc> rows;
+    long long gtot = 0;       // padded particles
+    long long iside_len = 0;  // floats
+    long long jside_len = 0;  // floats
+    long long chunk_len = 0;  // j per task of the mutual pass (0 if none)
 };
 
 // Scratch and plan of one owner (the drop-in entry points of a device, or
 // one ensemble).
 struct Workspace {
     DevBuf<float4> partial;
+    DevBuf<float4> gpos, gsum;     // mutual pass: padded positions, per-particle sums
+    DevBuf<float> iside, jside;    // mutual pass: row slots, j-side slabs
+    DevBuf<MutualTask> mtasks;
+    DevBuf<RowDesc> rows;
     DevBuf<int> counts, starts, sorted, cellhash, block_sums;
     DevBuf<float4> spos, svel;  // positions / velocities in bucket order
     DevBuf<GravTask> tasks;
@@ -131,8 +146,8 @@ struct DevCtx {
     hipEvent_t ev_in = nullptr, ev_out = nullptr;
     std::vector<GraphEntry> graphs;
     std::vector<TimingPair> pending, pool;
-    long long timed_launches[kTimedKinds] = {0, 0};
-    double timed_ms[kTimedKinds] = {0.0, 0.0};
+    long long timed_launches[kTimedKinds] = {0, 0, 0};
+    double timed_ms[kTimedKinds] = {0.0, 0.0, 0.0};
 };
 
 static std::mutex g_ctx_mutex;
@@ -184,8 +199,12 @@ static bool same_tiling(const Tiling& a, const Tiling& b) {
 // only (through its row of the tuning table), never on the rest of a batch:
 // chunk boundaries fix the order of every float sum, so a simulation gives
 // bitwise the same forces alone and inside any ensemble.
+// One-sided chunk length of a simulation of n particles planned alone (its
+// size's tuning row).  A simulation whose row asks for the mutual pass but
+// is too large for it takes the one-sided pass with the default task count.
 static long long sim_chunk_len(long long n) {
-    const Tiling t = choose_tiling(n, n * n, 1);
+    Tiling t = choose_tiling(n, n * n, 1);
+    if (t.variant == 2) t = Tiling{t.R, 0, 0, 32768, 512};
     long long L = t.chunk_len;
     if (L <= 0) {
         const long long ib = 64LL * t.R;
@@ -196,10 +215,94 @@ static long long sim_chunk_len(long long n) {
     return (L + 63) / 64 * 64;
 }
 
-// batched: every simulation uses the fixed chunk length tl.chunk_len (the
-// batched engine); otherwise each uses its own size's standalone rule.
+// j-side slabs of the mutual pass grow as 3 N^2 / (128 R) floats per
+// simulation; a simulation whose slab would exceed this many floats (about
+// N > 300,000 at R = 8) takes the one-sided pass instead.  The choice depends
+// on the simulation alone, so its sums are the same in any batch.
+constexpr long long kJsideMaxFloatsPerSim = 1LL << 28;
+
+// Mutual pass: padded layout, rows of B = 64 R, one diagonal task per row and
+// the row's j-range beyond it cut (at the iron/silicate border) into chunks
+// of at most L in whole 64-blocks.  Simulations left with gn = 0 (inactive,
+// empty or too large) are for the one-sided pass.
+static int plan_mutual(std::vector<SimDesc>& sims, const Tiling& tl, bool batched, Plan& p) {
+    const int B = 64 * tl.R;
+    long long gb = 0, jb = 0, off_total = 0, row = 0;
+    for (auto& s : sims) {
+        s.gbase = (int)gb;
+        s.row0 = (int)row;
+        s.jbase = jb;
+        s.gfe = s.gn = 0;
+        if (s.end == s.start || !s.active) continue;
+        const long long nfe = s.fe_end - s.start, nsi = s.end - s.fe_end;
+        const long long gfe = (nfe + B - 1) / B * B;
+        const long long gn = gfe + (nsi + B - 1) / B * B;
+        const long long nrows = gn / B;
+        if (3 * (gn * (nrows - 1) / 2) > kJsideMaxFloatsPerSim) continue;
+        s.gfe = (int)gfe;
+        s.gn = (int)gn;
+        gb += gn;
+        row += nrows;
+        off_total += gn * (nrows - 1) / 2;
+        jb += 3 * (gn * (nrows - 1) / 2);
+    }
+    if (gb == 0) return 0;
+    if (jb > (1LL << 33)) return set_error(GI_E_TOO_LARGE, "mutual pass: j-side slabs too large (%lld floats)", jb);
+    long long L = tl.chunk_len;
+    if (L <= 0 && !batched) {
+        const long long tt = tl.target_tasks > 0 ? tl.target_tasks : 8192;
+        L = (off_total + tt - 1) / tt;
+        if (L < tl.min_chunk) L = tl.min_chunk;
+    }
+    L = std::max(64LL, (L + 63) / 64 * 64);
+    std::vector<MutualTask> tasks;
+    std::vector<RowDesc> rows;
+    int slot = 0;
+    for (size_t si = 0; si < sims.size(); ++si) {
+        const SimDesc& s = sims[si];
+        if (s.gn == 0) continue;
+        const int nrows = s.gn / B;
+        for (int I = 0; I < nrows; ++I) {
+            RowDesc rd{slot, 0};
+            const int rb = I * B;
+            tasks.push_back(MutualTask{(int)si, I, rb, rb + B, slot++, rb < s.gfe ? 1 : 0, 1, 0});
+            const int seg0[2] = {rb + B, std::max(rb + B, s.gfe)};
+            const int seg1[2] = {s.gfe, s.gn};
+            for (int g = 0; g < 2; ++g) {
+                const int a0 = seg0[g], a1 = seg1[g];
+                if (a1 <= a0) continue;
+                const long long blocks = (a1 - a0) / 64;
+                long long pieces = (a1 - a0 + L - 1) / L;
+                if (pieces > blocks) pieces = blocks;
+                for (long long k = 0; k < pieces; ++k) {
+                    const int a = a0 + (int)(64 * (blocks * k / pieces));
+                    const int b = a0 + (int)(64 * (blocks * (k + 1) / pieces));
+                    tasks.push_back(MutualTask{(int)si, I, a, b, slot++, g == 0 ? 1 : 0, 0, 0});
+                }
+            }
+            rd.nslot = slot - rd.slot0;
+            rows.push_back(rd);
+        }
+    }
+    // Longest first (VALU ops per lane: R*12 per j one-sided, R*15+6 mutual).
+    auto cost = [&](const MutualTask& t) {
+        return (long long)(t.j1 - t.j0) * (t.diag ? 12 * tl.R : 15 * tl.R + 6);
+    };
+    std::stable_sort(tasks.begin(), tasks.end(),
+                     [&](const MutualTask& a, const MutualTask& b) { return cost(a) > cost(b); });
+    p.mtasks = std::move(tasks);
+    p.rows = std::move(rows);
+    p.gtot = gb;
+    p.iside_len = (long long)slot * 3 * B;
+    p.jside_len = jb;
+    p.chunk_len = L;
+    p.mutual = !p.mtasks.empty();
+    return 0;
+}
+
 static int build_plan(std::vector<SimDesc> sims, const Tiling& tl, bool batched, Plan* out) {
     Plan p;
+    p.caller_sims = sims;
     p.tiling = tl;
     long long ntot = 0;
     const int ib = 64 * tl.R;
@@ -211,14 +314,18 @@ static int build_plan(std::vector<SimDesc> sims, const Tiling& tl, bool batched,
     }
     // The contact-grid scan handles up to 4096 * 1024 buckets = 2 * 2^21.
     if (ntot > (1LL << 21)) return set_error(GI_E_TOO_LARGE, "too many particles (%lld > 2^21)", ntot);
+    if (tl.variant == 2) {
+        int rc = plan_mutual(sims, tl, batched, p);
+        if (rc < 0) return rc;
+    }
     long long pbase = 0;
     for (size_t si = 0; si < sims.size(); ++si) {
         SimDesc& s = sims[si];
         const int n = s.end - s.start;
         s.partial_base = (int)pbase;
         s.nchunk = 0;
-        if (n == 0 || !s.active) continue;
-        const long long L = batched ? tl.chunk_len : sim_chunk_len(n);
+        if (n == 0 || !s.active || s.gn > 0) continue;  // gn > 0: mutual pass
+        const long long L = batched ? (tl.variant == 2 ? 16384 : tl.chunk_len) : sim_chunk_len(n);
         // j segments: iron [start, fe_end), silicate [fe_end, end); each split
         // into near-equal chunks of at most L.
         const int seg0[2] = {s.start, s.fe_end};
@@ -297,30 +404,21 @@ static int ensure_plan(Workspace* ws, const std::vector<SimDesc>& sims, hipStrea
     Tiling tl = choose_tiling(ntot, pairs, (int)sims.size());
     if (ws->batched) {
         const BatchedTiling bt = batched_tiling();
-        tl = Tiling{bt.R, 0, bt.chunk_len, 0, 0};
+        tl = Tiling{bt.R, bt.variant, bt.chunk_len, 0, 0};
     }
-    if (ws->plan_valid && same_sims(ws->plan.sims, sims) && same_tiling(ws->plan.tiling, tl)) {
-        // same simulations; only partial_base/nchunk were filled in by the planner
-        return 0;
-    }
-    // same_sims compares against the planned copy, whose nchunk/partial_base
-    // are filled in: compare on the caller's fields only.
-    if (ws->plan_valid && same_tiling(ws->plan.tiling, tl) && ws->plan.sims.size() == sims.size()) {
-        bool same = true;
-        for (size_t k = 0; k < sims.size() && same; ++k) {
-            SimDesc a = sims[k], b = ws->plan.sims[k];
-            a.nchunk = b.nchunk;
-            a.partial_base = b.partial_base;
-            a.hbase = b.hbase;
-            a.hmask = b.hmask;
-            same = std::memcmp(&a, &b, sizeof a) == 0;
-        }
-        if (same) return 0;
-    }
+    if (ws->plan_valid && same_tiling(ws->plan.tiling, tl) && same_sims(ws->plan.caller_sims, sims)) return 0;
     Plan p;
     int rc = build_plan(sims, tl, ws->batched, &p);
     if (rc) return rc;
     if ((rc = ws->partial.ensure((size_t)p.partial_len + 1))) return rc;
+    if (p.mutual) {
+        if ((rc = ws->gpos.ensure((size_t)p.gtot + 1))) return rc;
+        if ((rc = ws->gsum.ensure((size_t)p.ntot + 1))) return rc;
+        if ((rc = ws->iside.ensure((size_t)p.iside_len + 1))) return rc;
+        if ((rc = ws->jside.ensure((size_t)p.jside_len + 1))) return rc;
+        if ((rc = upload(ws->mtasks, p.mtasks, st))) return rc;
+        if ((rc = upload(ws->rows, p.rows, st))) return rc;
+    }
     if ((rc = ws->counts.ensure((size_t)p.buckets + 1))) return rc;
     if ((rc = ws->starts.ensure((size_t)p.buckets + 2))) return rc;
     if ((rc = ws->spos.ensure((size_t)p.ntot + 1))) return rc;
@@ -404,28 +502,74 @@ static void launch_gravity_kernel(int R, dim3 grid, dim3 block, hipStream_t st, 
                            ws->partial.p);
 }
 
+template <bool POT>
+static void launch_mutual_kernel(int R, dim3 grid, dim3 block, hipStream_t st, const Workspace* ws, int ntasks) {
+    if (R == 8)
+        hipLaunchKernelGGL((k_mutual<8, POT>), grid, block, 0, st, ws->gpos.p, ws->mtasks.p, ntasks, ws->sims.p,
+                           ws->iside.p, ws->jside.p);
+    else if (R == 4)
+        hipLaunchKernelGGL((k_mutual<4, POT>), grid, block, 0, st, ws->gpos.p, ws->mtasks.p, ntasks, ws->sims.p,
+                           ws->iside.p, ws->jside.p);
+    else if (R == 2)
+        hipLaunchKernelGGL((k_mutual<2, POT>), grid, block, 0, st, ws->gpos.p, ws->mtasks.p, ntasks, ws->sims.p,
+                           ws->iside.p, ws->jside.p);
+    else
+        hipLaunchKernelGGL((k_mutual<1, POT>), grid, block, 0, st, ws->gpos.p, ws->mtasks.p, ntasks, ws->sims.p,
+                           ws->iside.p, ws->jside.p);
+}
+
+// The all-pairs pass of the current plan: the one-sided kernel for the
+// simulations planned one-sided, and for the others the mutual pass (padded
+// copy, pair kernel, per-particle sums into ws->gsum).  Timed: kind 0 spans
+// the all-pairs kernel(s), kind 2 the whole pass.
 static int launch_gravity(DevCtx* ctx, Workspace* ws, const float4* pos, bool pot, hipStream_t st,
                           bool timed) {
     const Plan& p = ws->plan;
-    const int ntasks = (int)p.tasks.size();
-    if (ntasks == 0) return 0;
-    const int wpb = 4;
-    dim3 grid(nblk(ntasks, wpb)), block(64 * wpb);
-    TimingPair tp;
+    const int nplain = (int)p.tasks.size();
+    const int nmut = (int)p.mtasks.size();
+    if (nplain == 0 && nmut == 0) return 0;
+    const int nsims = (int)p.sims.size();
+    TimingPair pass, tp;
+    if (timed) {
+        pass = take_pair(ctx, 2);
+        GI_HIP(hipEventRecord(pass.a, st));
+    }
+    if (nmut > 0)
+        hipLaunchKernelGGL(k_mutual_pad, dim3(nblk(p.gtot, 256)), dim3(256), 0, st, pos, (int)p.gtot, ws->sims.p,
+                           nsims, ws->gpos.p);
     if (timed) {
         tp = take_pair(ctx, 0);
         GI_HIP(hipEventRecord(tp.a, st));
     }
-    if (pot)
-        launch_gravity_kernel<true, 0>(p.tiling.R, grid, block, st, pos, ws, ntasks);
-    else if (p.tiling.variant == 1)
-        launch_gravity_kernel<false, 1>(p.tiling.R, grid, block, st, pos, ws, ntasks);
-    else
-        launch_gravity_kernel<false, 0>(p.tiling.R, grid, block, st, pos, ws, ntasks);
+    if (nplain > 0) {
+        const dim3 grid(nblk(nplain, 4)), block(256);
+        if (pot)
+            launch_gravity_kernel<true, 0>(p.tiling.R, grid, block, st, pos, ws, nplain);
+        else if (p.tiling.variant == 1)
+            launch_gravity_kernel<false, 1>(p.tiling.R, grid, block, st, pos, ws, nplain);
+        else
+            launch_gravity_kernel<false, 0>(p.tiling.R, grid, block, st, pos, ws, nplain);
+    }
+    if (nmut > 0) {
+        const dim3 grid(nblk(nmut, kMutualWaves)), block(64 * kMutualWaves);
+        if (pot)
+            launch_mutual_kernel<true>(p.tiling.R, grid, block, st, ws, nmut);
+        else
+            launch_mutual_kernel<false>(p.tiling.R, grid, block, st, ws, nmut);
+    }
     GI_HIP(hipGetLastError());
     if (timed) {
         GI_HIP(hipEventRecord(tp.b, st));
         ctx->pending.push_back(tp);
+    }
+    if (nmut > 0) {
+        hipLaunchKernelGGL(k_mutual_sum, dim3(nblk(p.ntot, 256)), dim3(256), 0, st, p.ntot, ws->sims.p, nsims,
+                           ws->rows.p, 64 * p.tiling.R, ws->iside.p, ws->jside.p, ws->gsum.p, pot ? 1 : 0);
+        GI_HIP(hipGetLastError());
+    }
+    if (timed) {
+        GI_HIP(hipEventRecord(pass.b, st));
+        ctx->pending.push_back(pass);
     }
     return 0;
 }
@@ -439,8 +583,8 @@ static int run_force(DevCtx* ctx, Workspace* ws, const float4* pos, const float4
     if (rc) return rc;
     if ((rc = launch_gravity(ctx, ws, pos, false, st, timed))) return rc;
     hipLaunchKernelGGL(k_finish, dim3(nblk(p.ntot, 256)), dim3(256), 0, st, force, p.ntot, ws->sims.p,
-                       (int)p.sims.size(), ws->partial.p, ws->starts.p, ws->sorted.p, ws->spos.p, ws->svel.p,
-                       p.tiling.variant);
+                       (int)p.sims.size(), ws->partial.p, p.mutual ? ws->gsum.p : nullptr, ws->starts.p,
+                       ws->sorted.p, ws->spos.p, ws->svel.p, p.tiling.variant);
     GI_HIP(hipGetLastError());
     return 0;
 }
@@ -453,7 +597,8 @@ static int run_conserved(DevCtx* ctx, Workspace* ws, const float4* pos, const fl
     if (rc) return rc;
     if ((rc = launch_gravity(ctx, ws, pos, true, st, false))) return rc;
     hipLaunchKernelGGL(k_finish_potential, dim3(nblk(p.ntot, 256)), dim3(256), 0, st, p.ntot, ws->sims.p,
-                       (int)p.sims.size(), ws->partial.p, ws->starts.p, ws->sorted.p, ws->spos.p, ws->pe.p);
+                       (int)p.sims.size(), ws->partial.p, p.mutual ? ws->gsum.p : nullptr, ws->starts.p,
+                       ws->sorted.p, ws->spos.p, ws->pe.p);
     const int nb = (int)p.block_sim.size();
     hipLaunchKernelGGL(k_moments, dim3(nb), dim3(256), 0, st, pos, vel, (const double*)ws->pe.p,
                        ws->block_sim.p, ws->block_first.p, ws->sims.p, 0.0, 0.0, ws->moments.p);
@@ -681,20 +826,46 @@ int gi_device_sync(void) {
     return 0;
 }
 
+int gi_tiling_for(long long n, int* r, int* variant, long long* chunk_len, long long* target_tasks,
+                  long long* min_chunk) {
+    if (n < 0) return set_error(GI_E_INVALID_ARG, "gi_tiling_for: n < 0");
+    const Tiling t = choose_tiling(n, n * n, 1);
+    if (r) *r = t.R;
+    if (variant) *variant = t.variant;
+    if (chunk_len) *chunk_len = t.chunk_len;
+    if (target_tasks) *target_tasks = t.target_tasks;
+    if (min_chunk) *min_chunk = t.min_chunk;
+    return 0;
+}
+
+int gi_plan_info(int* mutual, int* r, long long* pair_tasks, long long* chunk_len) {
+    DevCtx* ctx;
+    int rc = current_ctx(&ctx);
+    if (rc) return rc;
+    const Plan& p = ctx->ws.plan;
+    const bool valid = ctx->ws.plan_valid;
+    if (mutual) *mutual = valid && p.mutual ? 1 : 0;
+    if (r) *r = valid ? p.tiling.R : 0;
+    if (pair_tasks) *pair_tasks = valid ? (long long)(p.tasks.size() + p.mtasks.size()) : 0;
+    if (chunk_len) *chunk_len = valid ? p.chunk_len : 0;
+    return 0;
+}
+
 int gi_mem_info(size_t* free_bytes, size_t* total_bytes) {
     if (!free_bytes || !total_bytes) return set_error(GI_E_INVALID_ARG, "gi_mem_info: null pointer");
     GI_HIP(hipMemGetInfo(free_bytes, total_bytes));
     return 0;
 }
 
-int gi_set_batched_tiling(int r, long long chunk_len) {
+int gi_set_batched_tiling(int r, long long chunk_len, int variant) {
     if (r == 0) {
-        batched_override() = BatchedTiling{0, 0};
+        batched_override() = BatchedTiling{0, 0, 0};
         return 0;
     }
-    if ((r != 1 && r != 2 && r != 4 && r != 8) || chunk_len < 64)
-        return set_error(GI_E_INVALID_ARG, "gi_set_batched_tiling: r in {1,2,4,8} and chunk_len >= 64");
-    batched_override() = BatchedTiling{(chunk_len + 63) / 64 * 64, r};
+    if ((r != 1 && r != 2 && r != 4 && r != 8) || chunk_len < 64 || variant < 0 || variant > 2)
+        return set_error(GI_E_INVALID_ARG,
+                         "gi_set_batched_tiling: r in {1,2,4,8}, chunk_len >= 64, variant in {0,1,2}");
+    batched_override() = BatchedTiling{(chunk_len + 63) / 64 * 64, r, variant};
     return 0;
 }
 
@@ -705,7 +876,7 @@ int gi_set_tiling(int r, int variant, long long chunk_len, long long target_task
     }
     if (r != 1 && r != 2 && r != 4 && r != 8)
         return set_error(GI_E_INVALID_ARG, "gi_set_tiling: r must be 1, 2, 4 or 8");
-    if (variant != 0 && variant != 1) return set_error(GI_E_INVALID_ARG, "gi_set_tiling: variant must be 0 or 1");
+    if (variant < 0 || variant > 2) return set_error(GI_E_INVALID_ARG, "gi_set_tiling: variant must be 0, 1 or 2");
     if (chunk_len < 0 || target_tasks < 0 || min_chunk < 0)
         return set_error(GI_E_INVALID_ARG, "gi_set_tiling: negative size");
     tiling_override().active = 1;
@@ -1097,8 +1268,10 @@ int gi_kernel_timing(const char* kernel, long long* launches, double* total_ms) 
     int kind = 0;
     if (kernel && std::strcmp(kernel, "move") == 0)
         kind = 1;
+    else if (kernel && std::strcmp(kernel, "gravity_pass") == 0)
+        kind = 2;
     else if (kernel && std::strcmp(kernel, "gravity") != 0)
-        return set_error(GI_E_INVALID_ARG, "unknown kernel '%s' (timed kernels: gravity, move)", kernel);
+        return set_error(GI_E_INVALID_ARG, "unknown kernel '%s' (timed: gravity, gravity_pass, move)", kernel);
     DevCtx* ctx;
     int rc = current_ctx(&ctx);
     if (rc) return rc;

@@ -29,7 +29,31 @@ struct SimDesc {
     PairLaw law;
     float dt, dt_over_mass_fe, dt_over_mass_si;
     float pad2;
+    // Mutual all-pairs pass: the simulation's copy in the padded layout
+    // (iron then silicate, each padded with ghosts to a whole number of
+    // 64*R rows), its first row in the row table and its j-side slab.
+    int gbase, gfe, gn, row0;
+    long long jbase;
 };
+
+// One wavefront of the mutual all-pairs pass: row `row` (64*R resident
+// particles of the padded layout) against j in [j0, j1) of the same
+// simulation, all of material jfe.  diag: the row against itself, one-sided;
+// otherwise j lies beyond the row and each pair is evaluated once for both.
+struct MutualTask {
+    int sim, row, j0, j1, slot, jfe, diag, pad;
+};
+
+// The i-side slots of one row: slot0 .. slot0 + nslot - 1, in task order.
+struct RowDesc {
+    int slot0, nslot;
+};
+
+// Ghost position of the padded layout: every pair with a real particle has
+// r^-3 = 0 in float (r^2 ~ 3e36), and a ghost pair has dp = 0.
+constexpr float kGhostCoord = 1e18f;
+// Any r^2 at or above this is a ghost pair (real separations are far below).
+constexpr float kGhostR2 = 1e30f;
 
 // One wavefront of work for the all-pairs pass: 64*R i-particles starting at
 // i0 of simulation sim, against j in [j0, j1) (all of one material: jfe).

@@ -1,8 +1,9 @@
 // gi_tuning.h -- tiling of the all-pairs pass and its tuning table.
 //
 // R            i-particles held in registers per lane (a wavefront covers 64*R)
-// variant      how r^-3 is formed: 0 = rsq cubed (3 multiplies' worth of
-//              VALU), 1 = rsq * rcp (1 multiply, 2 transcendental ops)
+// variant      0 = one-sided pass, r^-3 as rsq cubed; 1 = one-sided, rsq *
+//              rcp (1 multiply, 2 transcendental ops); 2 = mutual pass (each
+//              pair once for both particles, rows of 64*R, rsq cubed)
 // chunk_len    j-particles per task (0 = derive from target_tasks)
 // target_tasks wavefront tasks to aim for (several per SIMD of the 256 CUs x
 //              4 SIMDs so the tail of the launch stays short)
@@ -72,8 +73,8 @@ inline const std::vector<TuningRow>& tuning_table() {
             TuningRow r;
             if (std::sscanf(line, "%lld %d %d %lld %lld %lld", &r.n_max, &r.t.R, &r.t.variant, &r.t.chunk_len,
                             &r.t.target_tasks, &r.t.min_chunk) == 6 &&
-                (r.t.R == 1 || r.t.R == 2 || r.t.R == 4 || r.t.R == 8) && (r.t.variant == 0 || r.t.variant == 1) &&
-                r.t.target_tasks > 0)
+                (r.t.R == 1 || r.t.R == 2 || r.t.R == 4 || r.t.R == 8) && r.t.variant >= 0 && r.t.variant <= 2 &&
+                r.t.chunk_len >= 0 && r.t.target_tasks >= 0 && (r.t.target_tasks > 0 || r.t.chunk_len > 0))
                 rows.push_back(r);
         }
         std::fclose(f);
@@ -83,21 +84,22 @@ inline const std::vector<TuningRow>& tuning_table() {
 
 // Tiling of the batched engine (ensembles and the request loop): one j-chunk
 // length for every simulation, whatever its size or company, so that a
-// simulation's sums are the same in any batch.  Row "B chunk_len R" of the
-// tuning file; gi_set_batched_tiling overrides it.
+// simulation's sums are the same in any batch.  Row "B chunk_len R [variant]"
+// of the tuning file; gi_set_batched_tiling overrides it.
 struct BatchedTiling {
     long long chunk_len;
     int R;
+    int variant;
 };
 
 inline BatchedTiling& batched_override() {
-    static BatchedTiling b{0, 0};
+    static BatchedTiling b{0, 0, 0};
     return b;
 }
 
 inline BatchedTiling batched_tiling() {
     if (batched_override().chunk_len > 0) return batched_override();
-    static BatchedTiling from_file{0, 0};
+    static BatchedTiling from_file{0, 0, 0};
     static std::once_flag once;
     std::call_once(once, [] {
         Dl_info info;
@@ -113,14 +115,15 @@ inline BatchedTiling batched_tiling() {
         char line[256];
         while (std::fgets(line, sizeof line, f)) {
             long long L;
-            int R;
-            if (std::sscanf(line, "B %lld %d", &L, &R) == 2 && L >= 64 && (R == 1 || R == 2 || R == 4 || R == 8))
-                from_file = BatchedTiling{L, R};
+            int R, V = 0;
+            const int got = std::sscanf(line, "B %lld %d %d", &L, &R, &V);
+            if (got >= 2 && L >= 64 && (R == 1 || R == 2 || R == 4 || R == 8) && V >= 0 && V <= 2)
+                from_file = BatchedTiling{L, R, V};
         }
         std::fclose(f);
     });
     if (from_file.chunk_len > 0) return from_file;
-    return BatchedTiling{4096, 1};
+    return BatchedTiling{4096, 1, 0};
 }
 
 inline Tiling choose_tiling(long long ntot, long long pairs, int nsims) {

@@ -258,7 +258,9 @@ def load() -> C.CDLL:
         "gi_ensemble_conserved": (i, [vp, P(C.c_double)]),
         "gi_ensemble_work": (i, [vp, P(C.c_longlong), P(C.c_double)]),
         "gi_set_tiling": (i, [i, i, C.c_longlong, C.c_longlong, C.c_longlong]),
-        "gi_set_batched_tiling": (i, [i, C.c_longlong]),
+        "gi_plan_info": (i, [P(i), P(i), P(C.c_longlong), P(C.c_longlong)]),
+        "gi_tiling_for": (i, [C.c_longlong, P(i), P(i), P(C.c_longlong), P(C.c_longlong), P(C.c_longlong)]),
+        "gi_set_batched_tiling": (i, [i, C.c_longlong, i]),
         "gi_serve_run": (i, [vp, i, vp, vp, vp]),
         "gi_spin_body": (i, [vp, vp, i, i, i, C.c_double, C.c_double, C.c_double, vp]),
         "gi_zero_out_drift": (i, [vp, vp, vp]),
@@ -429,6 +431,22 @@ def force_separate(pos, vel, force, n: int, fc: ForceSeparateConstants, stream=N
 
 def move_damped_separate(pos, vel, force, n: int, mc: MoveSeparateConstants, d1: float, d2: float, stream=None):
     call("gi_move_damped_separate", _p(pos), _p(vel), _p(force), int(n), C.byref(mc), float(d1), float(d2), _p(stream))
+
+
+def tiling_for(n: int) -> dict:
+    """The tiling a single simulation of n particles gets (host only)."""
+    r, v = C.c_int(), C.c_int()
+    L, t, m = C.c_longlong(), C.c_longlong(), C.c_longlong()
+    call("gi_tiling_for", int(n), C.byref(r), C.byref(v), C.byref(L), C.byref(t), C.byref(m))
+    return {"r": r.value, "variant": v.value, "chunk_len": L.value, "target_tasks": t.value, "min_chunk": m.value}
+
+
+def plan_info() -> dict:
+    """The plan of the last drop-in force call on the current device."""
+    m, r, t, L = C.c_int(), C.c_int(), C.c_longlong(), C.c_longlong()
+    call("gi_plan_info", C.byref(m), C.byref(r), C.byref(t), C.byref(L))
+    return {"mutual": bool(m.value), "r": r.value, "pair_tasks": t.value, "chunk_len": L.value,
+            "kernel": "k_mutual" if m.value else "k_gravity"}
 
 
 def sync() -> None:

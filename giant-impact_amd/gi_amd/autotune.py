@@ -1,9 +1,11 @@
 """Autotuner of the all-pairs pass (run on the GPU).
 
-For each size class it times the all-pairs gravity kernel (HIP events, via
-gi_kernel_timing) over a grid of tilings -- R (i-particles per lane), the
-r^-3 variant and the number of wavefront tasks -- and writes the fastest to
-tuning/gfx950.txt, which libgiant_impact.so reads on first use:
+For each size class it times the whole all-pairs pass (HIP events, via
+gi_kernel_timing("gravity_pass")) over a grid of tilings -- the one-sided
+pass with R (i-particles per lane), the r^-3 variant and the number of
+wavefront tasks, and the mutual pass (variant 2) with R (rows of 64*R) and
+its chunk length -- and writes the fastest to tuning/gfx950.txt, which
+libgiant_impact.so reads on first use:
 
     n_max R variant chunk_len target_tasks min_chunk
 
@@ -50,15 +52,15 @@ def time_gravity(n_target: int, n_impactor: int, reps: int):
         sync()
         lib.gi_set_kernel_timing(0)
         L, T = C.c_longlong(), C.c_double()
-        call("gi_kernel_timing", b"gravity", C.byref(L), C.byref(T))
+        call("gi_kernel_timing", b"gravity_pass", C.byref(L), C.byref(T))
         return T.value / max(L.value, 1)
 
     return run, n
 
 
 def tune_batched(quick: bool):
-    """Chunk length and R of the batched engine, timed on the 64-simulation
-    uneven ensemble (N uniform in [4096, 32768])."""
+    """Pass, chunk length and R of the batched engine, timed on the
+    64-simulation uneven ensemble (N uniform in [4096, 32768])."""
     from . import Ensemble, ensemble_configs, make_ensemble
 
     pos, vel, n, fc, mc, _ = make_ensemble(ensemble_configs(64))
@@ -67,8 +69,10 @@ def tune_batched(quick: bool):
     P, V = DeviceArray(ntot, pos), DeviceArray(ntot, vel)
     best = None
     Ls = [1024, 2048, 4096, 8192, 16384] if not quick else [4096]
-    for R, L in itertools.product([1, 2, 4], Ls):
-        call("gi_set_batched_tiling", R, L)
+    cands = [(R, L, 0) for R, L in itertools.product([1, 2], Ls)]
+    cands += [(R, L, 2) for R, L in itertools.product([4, 8], [256, 512, 1024, 2048])]
+    for R, L, v in cands:
+        call("gi_set_batched_tiling", R, L, v)
         P.upload(pos)
         V.upload(vel)
         e = Ensemble(P, V, n, fc, mc)
@@ -79,12 +83,13 @@ def tune_batched(quick: bool):
         sync()
         dt = (time.perf_counter() - t0) / 10
         e.close()
-        print(f"batched R={R} L={L:6d}: {dt * 1e3:8.3f} ms/step  {pairs / dt:.3e} pair/s", flush=True)
+        print(f"batched R={R} v={v} L={L:6d}: {dt * 1e3:8.3f} ms/step  {pairs / dt:.3e} pair/s", flush=True)
         if best is None or dt < best[0]:
-            best = (dt, L, R)
-    call("gi_set_batched_tiling", 0, 0)
-    print(f"# batched: best chunk_len={best[1]} R={best[2]} ({pairs / best[0]:.3e} pair/s)", flush=True)
-    return best[1], best[2]
+            best = (dt, L, R, v)
+    call("gi_set_batched_tiling", 0, 0, 0)
+    print(f"# batched: best chunk_len={best[1]} R={best[2]} variant={best[3]} ({pairs / best[0]:.3e} pair/s)",
+          flush=True)
+    return best[1], best[2], best[3]
 
 
 def main(argv=None) -> int:
@@ -104,20 +109,26 @@ def main(argv=None) -> int:
     # at N = 262,144); --variants sweeps it again
     variants = [0, 1] if args.variants else [0]
     targets = [4096, 8192, 16384, 32768, 65536, 131072] if not args.quick else [32768]
+    mutual_Ls = [256, 512, 1024, 2048] if not args.quick else [512]
     rows = []
     for n_max, nt, ni in CLASSES[: args.max_class]:
         run, n = time_gravity(nt, ni, reps=3 if nt + ni > 100000 else 10)
+        mc = 128 if n < 4096 else 512
+        # (R, variant, chunk_len, target_tasks, min_chunk)
+        cands = [(R, v, 0, tt, mc) for R, v, tt in itertools.product(Rs, variants, targets)]
+        cands += [(R, 2, L, 0, 64) for R, L in itertools.product([4, 8], mutual_Ls)]
         best = None
-        for R, v, tt in itertools.product(Rs, variants, targets):
-            call("gi_set_tiling", R, v, 0, tt, 128 if n < 4096 else 512)
+        for cand in cands:
+            call("gi_set_tiling", *cand)
             ms = run()
             pairs = n * n / (ms / 1e3)
-            print(f"n={n:8d} R={R} v={v} tasks={tt:6d}: {ms:9.4f} ms  {pairs:.3e} pair/s", flush=True)
+            print(f"n={n:8d} R={cand[0]} v={cand[1]} L={cand[2]:5d} tasks={cand[3]:6d}: {ms:9.4f} ms  "
+                  f"{pairs:.3e} pair/s", flush=True)
             if best is None or ms < best[0]:
-                best = (ms, R, v, tt)
-        ms, R, v, tt = best
-        print(f"# class n<={n_max}: best R={R} variant={v} tasks={tt} ({n * n / (ms / 1e3):.3e} pair/s)", flush=True)
-        rows.append((n_max, R, v, 0, tt, 128 if n < 4096 else 512))
+                best = (ms, cand)
+        ms, cand = best
+        print(f"# class n<={n_max}: best {cand} ({n * n / (ms / 1e3):.3e} pair/s)", flush=True)
+        rows.append((n_max,) + cand)
     call("gi_set_tiling", 0, 0, 0, 0, 0)
     batched = tune_batched(args.quick) if args.batched else None
     out = Path(args.out)
@@ -128,8 +139,8 @@ def main(argv=None) -> int:
         for r in rows:
             f.write(" ".join(str(x) for x in r) + "\n")
         if batched:
-            f.write("# batched engine: B chunk_len R\n")
-            f.write(f"B {batched[0]} {batched[1]}\n")
+            f.write("# batched engine: B chunk_len R variant\n")
+            f.write(f"B {batched[0]} {batched[1]} {batched[2]}\n")
     print(f"wrote {out}")
     return 0
 

@@ -149,6 +149,12 @@ int gi_stream_sync(gi_stream_t stream);
 int gi_device_sync(void);
 /* Free and total device memory in bytes (hipMemGetInfo). */
 int gi_mem_info(size_t *free_bytes, size_t *total_bytes);
+/* The plan of the last drop-in force call on the current device: mutual = 1
+ * when it uses the mutual all-pairs pass (k_mutual) for some simulation, r =
+ * i-particles per lane, pair_tasks = wavefront tasks of the all-pairs
+ * kernel(s), chunk_len = j per task of the mutual pass.  Zeros before the
+ * first call.  Any pointer may be NULL. */
+int gi_plan_info(int *mutual, int *r, long long *pair_tasks, long long *chunk_len);
 
 /* ---- the four step-loop kernels ----------------------------------------- */
 
@@ -247,20 +253,29 @@ int gi_ensemble_work(gi_ensemble *e, long long *particles, double *pairs);
 
 /* Override the tiling chosen from the tuning table (tuning/gfx950.txt beside
  * the library's lib/ directory, written by the autotuner): r = i-particles per
- * lane (1, 2, 4 or 8), variant = how r^-3 is formed (0: rsq cubed, 1: rsq*rcp),
- * chunk_len = j-particles per task (0 = derived from target_tasks),
+ * lane (1, 2, 4 or 8), variant = the all-pairs pass (0: one-sided, r^-3 as rsq
+ * cubed; 1: one-sided, rsq*rcp; 2: mutual -- each pair evaluated once for both
+ * particles, rows of 64*r particles), chunk_len = j-particles per task (0 =
+ * derived from target_tasks),
  * target_tasks = wavefront tasks to aim for, min_chunk = lower bound of
  * chunk_len.  r = 0 restores the table.  Plans built with another tiling are
  * rebuilt on their next use. */
 int gi_set_tiling(int r, int variant, long long chunk_len, long long target_tasks, long long min_chunk);
 
+/* The tiling a single simulation of n particles gets (override, else the
+ * tuning table's row, else the built-in default).  Host only: needs no
+ * device.  Any output pointer may be NULL. */
+int gi_tiling_for(long long n, int *r, int *variant, long long *chunk_len, long long *target_tasks,
+                  long long *min_chunk);
+
 /* Tiling of the batched engine (ensembles, request loop): every simulation's
  * j-range is cut into chunks of chunk_len whatever the batch, so a simulation
  * computes bitwise the same alone (an ensemble of one) and in any mix; r =
- * i-particles per lane of the launch.  Default: row "B chunk_len r" of the
- * tuning file.  r = 0 restores the default; takes effect at the next re-plan
- * (a new ensemble or a change of the active set). */
-int gi_set_batched_tiling(int r, long long chunk_len);
+ * i-particles per lane of the launch, variant as for gi_set_tiling.  Default:
+ * row "B chunk_len r [variant]" of the tuning file.  r = 0 restores the
+ * default; takes effect at the next re-plan (a new ensemble or a change of the
+ * active set). */
+int gi_set_batched_tiling(int r, long long chunk_len, int variant);
 
 /* ---- kernel timing (HIP events around the all-pairs force kernel) ------- */
 

@@ -1,0 +1,166 @@
+"""The mutual all-pairs pass (tiling variant 2): each pair evaluated once and
+applied to both particles, rows of 64*R particles in a padded layout with
+ghosts.  Checked at every row size against the oracle, against the one-sided
+pass, in the generator's two-body form, for the potential, for ragged and
+degenerate sizes (ghost padding, one material only), and in the batched
+engine (bitwise the same alone and in any mix).  Tolerances: tests/parity.py.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from conftest import f4
+from parity import assert_force_parity, force_metrics
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture
+def tiling(gpu):
+    def set_tiling(r, variant, chunk_len=0, target_tasks=0, min_chunk=64):
+        gpu.call("gi_set_tiling", r, variant, chunk_len, target_tasks, min_chunk)
+
+    yield set_tiling
+    gpu.call("gi_set_tiling", 0, 0, 0, 0, 0)
+
+
+@pytest.fixture
+def batched_mutual(gpu):
+    gpu.call("gi_set_batched_tiling", 8, 1024, 2)
+    yield
+    gpu.call("gi_set_batched_tiling", 0, 0, 0)
+
+
+def _force(gi, pos, vel, fc):
+    n = len(pos)
+    P, V, F = gi.DeviceArray(n, pos), gi.DeviceArray(n, vel), gi.DeviceArray(n, f4(n))
+    gi.force_collision(P, V, F, n, fc)
+    gi.sync()
+    return F.download()
+
+
+CASES = {
+    # name: (n_target, n_impactor, seed, angle, speed, target spin)
+    "impact_2048": (1536, 512, 20261202, 45.0, 1.2, 0.0),
+    "ragged_4097": (3000, 1097, 3, 60.0, 1.4, 0.3),
+    "small_770": (700, 70, 5, 30.0, 1.0, 0.2),
+}
+
+
+@pytest.mark.parametrize("R", [1, 2, 4, 8])
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_mutual_force_parity(gpu, oracle, tiling, case, R):
+    nt, ni, seed, ang, spd, spin = CASES[case]
+    pos, vel, fc, _, _ = gpu.make_impact(nt, ni, seed=seed, angle_deg=ang, speed_vesc=spd, target_spin=spin)
+    print(f"case={case} R={R}", flush=True)
+    tiling(R, 2, 128)  # short chunks: several tasks and j-blocks per row
+    F = _force(gpu, pos, vel, fc)
+    F64, cond, near = oracle.force_collision_f64(pos, vel, fc)
+    met = force_metrics(F, F64, cond, near, oracle.force_collision_f32(pos, vel, fc))
+    print(met)
+    assert_force_parity(met)
+    assert np.all(F[:, 3] == 0.0)
+
+
+def test_mutual_agrees_with_the_one_sided_pass(gpu, tiling):
+    pos, vel, fc, _, _ = gpu.make_impact(15000, 5000, seed=8, angle_deg=40.0, speed_vesc=1.1)
+    print("n=20000", flush=True)
+    tiling(1, 0, 0, 8192, 512)
+    Fp = _force(gpu, pos, vel, fc)[:, :3].astype(np.float64)
+    for R, L in ((4, 1024), (8, 2048)):
+        tiling(R, 2, L)
+        Fm = _force(gpu, pos, vel, fc)[:, :3].astype(np.float64)
+        d = np.linalg.norm(Fm - Fp, axis=1) / np.maximum(np.linalg.norm(Fp, axis=1), 1e-30)
+        print(R, L, d.max(), np.percentile(d, 99.9))
+        assert d.max() <= 5e-6 and np.percentile(d, 99.9) <= 1e-6
+    # Newton's third law over the whole system
+    tot = Fm.sum(0)
+    assert np.abs(tot).max() <= 1e-5 * np.abs(Fm).sum()
+
+
+@pytest.mark.parametrize("n,nfe", [(1, 0), (2, 1), (3, 3), (65, 0), (257, 100), (640, 640), (1100, 512)])
+def test_mutual_small_and_ragged_sizes(gpu, oracle, tiling, n, nfe):
+    """Rows padded with ghosts, one-material simulations, a material border
+    on a row boundary (nfe = 512 = 64*8)."""
+    rng = np.random.default_rng(n + 7)
+    pos, vel = f4(n), f4(n)
+    pos[:, :3] = rng.uniform(-0.6, 0.6, (n, 3)) * max(1.0, n ** (1 / 3))
+    vel[:, :3] = rng.normal(0, 2, (n, 3))
+    fc = gpu.make_impact(10, 0)[2]
+    fc.NFe = nfe
+    print(f"n={n} nfe={nfe}", flush=True)
+    tiling(8, 2, 64)
+    F = _force(gpu, pos, vel, fc)
+    F64, cond, near = oracle.force_collision_f64(pos, vel, fc)
+    if n == 1:
+        assert np.all(F[:, :3] == 0)
+        return
+    met = force_metrics(F, F64, cond, near, oracle.force_collision_f32(pos, vel, fc))
+    assert_force_parity(met, check_ref=False)
+
+
+def test_mutual_force_separate_parity(gpu, oracle, tiling):
+    """Two bodies overlapping in space: the cross-body pairs must be skipped."""
+    from test_gpu_force import _generator_bodies
+
+    pos, vel, fs, _ = _generator_bodies(gpu, 1200, 500, 21)
+    n = len(pos)
+    tiling(4, 2, 128)
+    P, V, F = gpu.DeviceArray(n, pos), gpu.DeviceArray(n, vel), gpu.DeviceArray(n, f4(n))
+    gpu.force_separate(P, V, F, n, fs)
+    gpu.sync()
+    Fg = F.download()
+    F64, cond, near = oracle.force_separate_f64(pos, vel, fs)
+    met = force_metrics(Fg, F64, cond, near, oracle.force_separate_f32(pos, vel, fs))
+    print(met)
+    assert_force_parity(met)
+
+
+def test_mutual_conserved_quantities_match_fp64_oracle(gpu, oracle, tiling):
+    pos, vel, fc, _, _ = gpu.make_impact(1536, 512, seed=5, target_spin=0.5)
+    n = len(pos)
+    P, V = gpu.DeviceArray(n, pos), gpu.DeviceArray(n, vel)
+    tiling(8, 2, 128)
+    out = (C.c_double * 9)()
+    gpu.call("gi_conserved_quantities", P.ptr, V.ptr, n, C.byref(fc), out, None)
+    g = np.array(list(out))
+    o = oracle.conserved_f64(pos, vel, fc)
+    print(g, o)
+    assert g[0] == pytest.approx(o[0], rel=1e-12)
+    assert g[1] == pytest.approx(o[1], rel=2e-6)
+
+
+def _ensemble_of_one(gi, pos, vel, fc, mc, steps):
+    n = len(pos)
+    P, V = gi.DeviceArray(n, pos), gi.DeviceArray(n, vel)
+    e = gi.Ensemble(P, V, np.array([n], np.int32), (gi.ForceCollisionConstants * 1)(fc),
+                    (gi.MoveCollisionConstants * 1)(mc))
+    e.steps(steps)
+    gi.sync()
+    out = P.download(), V.download()
+    e.close()
+    return out
+
+
+def test_mutual_batched_is_bitwise_the_same_alone_and_in_the_mix(gpu, batched_mutual):
+    cfg = gpu.ensemble_configs(5, seed=13, n_range=(700, 4100))
+    pos, vel, n, fc, mc, _ = gpu.make_ensemble(cfg)
+    print("n =", list(n), flush=True)
+    P, V = gpu.DeviceArray(len(pos), pos), gpu.DeviceArray(len(pos), vel)
+    ens = gpu.Ensemble(P, V, n, fc, mc)
+    ens.set_active([1, 1, 0, 1, 1])
+    ens.steps(3)
+    ens.set_active([1, 1, 1, 1, 1])
+    ens.steps(4)
+    gpu.sync()
+    p_all, v_all = P.download(), V.download()
+    off = ens.offsets
+    for s, k in zip(range(len(n)), (7, 7, 4, 7, 7)):
+        a, b = off[s], off[s + 1]
+        p1, v1 = _ensemble_of_one(gpu, pos[a:b], vel[a:b], fc[s], mc[s], k)
+        assert np.array_equal(p_all[a:b], p1), s
+        assert np.array_equal(v_all[a:b], v1), s
+    ens.close()

@@ -137,7 +137,7 @@ def test_invalid_arguments_are_rejected_before_touching_a_device(gi):
     assert lib.gi_force_separate(vp(1), vp(1), vp(1), 10, C.byref(fs), None) == -1
     assert lib.gi_move_collision(vp(1), vp(1), vp(1), -3, C.byref(mc), None) == -1
     assert lib.gi_check_block_multiple(1000) == -2 and lib.gi_check_block_multiple(1024) == 0
-    assert lib.gi_set_tiling(3, 0, 0, 0, 0) == -1 and lib.gi_set_tiling(4, 2, 0, 0, 0) == -1
+    assert lib.gi_set_tiling(3, 0, 0, 0, 0) == -1 and lib.gi_set_tiling(4, 3, 0, 0, 0) == -1
     assert lib.gi_set_tiling(0, 0, 0, 0, 0) == 0
     assert lib.gi_impact_steps(vp(1), vp(1), vp(1), 10, C.byref(fc), C.byref(mc), 5, 1, None, None, None) == -1
 
@@ -199,3 +199,25 @@ def test_zero_out_drift_uses_the_nominal_body_masses(gi, oracle):
         vm = oracle.body_com(vel, a, f, e, u.mass_fe, u.mass_si, M)[:3]
         assert np.array_equal(p[a:e, :3], (pos[a:e, :3].astype(np.float64) - com).astype(np.float32))
         assert np.array_equal(v[a:e, :3], (vel[a:e, :3].astype(np.float64) - vm).astype(np.float32))
+
+
+def test_every_tuning_row_is_accepted(gi):
+    """Each row of tuning/gfx950.txt reaches the planner: the tiling chosen
+    for a size at a row's n_max is that row's (a rejected row would fall
+    through to the next one)."""
+    rows = []
+    for line in (REPO / "giant-impact_amd" / "tuning" / "gfx950.txt").read_text().splitlines():
+        parts = line.split()
+        if not parts or parts[0].startswith("#") or parts[0] == "B":
+            continue
+        rows.append([int(x) for x in parts])
+    assert rows
+    for n_max, r, variant, chunk_len, target_tasks, min_chunk in rows:
+        t = gi.tiling_for(n_max)
+        assert (t["r"], t["variant"], t["chunk_len"], t["target_tasks"], t["min_chunk"]) == (
+            r, variant, chunk_len, target_tasks, min_chunk), (n_max, t)
+    gi.call("gi_set_tiling", 8, 2, 512, 0, 64)
+    try:
+        assert gi.tiling_for(1000)["variant"] == 2
+    finally:
+        gi.call("gi_set_tiling", 0, 0, 0, 0, 0)
