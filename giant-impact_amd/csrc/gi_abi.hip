@@ -216,10 +216,10 @@ static long long sim_chunk_len(long long n) {
 }
 
 // j-side slabs of the mutual pass grow as 3 N^2 / (128 R) floats per
-// simulation; a simulation whose slab would exceed this many floats (about
-// N > 300,000 at R = 8) takes the one-sided pass instead.  The choice depends
+// simulation; a simulation whose slab would exceed this many floats (2 GiB:
+// about N > 430,000 at R = 8, 300,000 at R = 4) takes the one-sided pass.  The choice depends
 // on the simulation alone, so its sums are the same in any batch.
-constexpr long long kJsideMaxFloatsPerSim = 1LL << 28;
+constexpr long long kJsideMaxFloatsPerSim = 1LL << 29;
 
 // Mutual pass: padded layout, rows of B = 64 R, one diagonal task per row and
 // the row's j-range beyond it cut (at the iron/silicate border) into chunks

@@ -142,13 +142,16 @@ __device__ __forceinline__ int find_sim(const SimDesc* __restrict__ sims, int ns
 // j receives from i is exactly the negation of the one i receives from j.
 // A wavefront holds a row of 64*R particles in registers (R per lane) and
 // streams 64-particle blocks j of the same simulation that lie beyond the
-// row.  Each lane also holds one j (position and a force accumulator); after
-// every step the j registers rotate by one lane (DPP wave_ror), so in 64
+// row.  Each lane holds the force accumulator of one j of the block; after
+// every step the accumulators rotate by one lane (DPP wave_ror), so in 64
 // steps every lane has met every j of the block and each j has collected the
-// terms of all 64*R row particles.  Per pair: 15 VALU ops including one rsq
-// plus 6/R lane rotations, for two interactions, against 12 ops per
-// interaction one-sided.  (Accumulating the j-side terms with LDS float adds
-// instead was measured about 5x slower on gfx950.)
+// terms of all 64*R row particles.  The positions stay put in wave-private
+// LDS and each step reads the one its lane's accumulator now belongs to.
+// Per pair: 15 VALU ops including one rsq plus 3/R lane rotations, for two
+// interactions, against 12 ops per interaction one-sided.  Measured on
+// gfx950: rotating the positions with DPP as well is 6% slower (the
+// rotation sits on every step's dependency chain); accumulating the j-side
+// terms with LDS float adds instead of rotating is about 5x slower.
 //
 // Simulations are copied into a padded layout (iron then silicate, each
 // rounded up to whole rows with ghosts far away) so that every row and every
@@ -161,6 +164,14 @@ __device__ __forceinline__ int find_sim(const SimDesc* __restrict__ sims, int ns
 __device__ __forceinline__ float rotate_lanes(float v) {
     // wave_ror:1 -- lane l reads lane l-1 (mod 64).
     return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x13C, 0xF, 0xF, false));
+}
+
+// Orders this wavefront's LDS accesses before and after (its lanes exchange
+// data through LDS).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 // Offset (in entries, per plane) of row `row`'s part of a simulation's
@@ -239,17 +250,29 @@ __global__ __launch_bounds__(64 * kMutualWaves) void k_mutual(const float4* __re
         const int first = (t.row + 1) * B;  // first j of the row's slab
         const long long len = sd.gn - first;
         float* __restrict__ J = jside + sd.jbase + 3 * jside_row_offset(t.row, sd.gn, B);
+        // The block's positions, twice, in wave-private LDS: q[m] holds
+        // j_((127 - m) mod 64), so step k reads q[63 - lane + k] =
+        // j_((lane - k) mod 64) -- the j whose accumulator lane `lane` holds
+        // after k rotations -- at the immediate offset 16 k.
+        __shared__ float4 lq[kMutualWaves][128];
+        float4* __restrict__ q = lq[wv];
         float4 nx = P[t.j0 + lane];
         for (int J0 = t.j0; J0 < t.j1; J0 += 64) {
             float xj = nx.x, yj = nx.y, zj = nx.z;
+            wave_lds_sync();  // the previous block's reads are done
+            q[63 - lane] = nx;
+            q[127 - lane] = nx;
+            wave_lds_sync();
             // next block's positions, in flight during this block
             nx = P[(J0 + 64 < t.j1 ? J0 + 64 : J0) + lane];
+            const float4* __restrict__ qb = q + (63 - lane);
             float fx = 0.0f, fy = 0.0f, fz = 0.0f;
             float tx[R], ty[R], tz[R];
 #pragma unroll
             for (int r = 0; r < R; ++r) tx[r] = ty[r] = tz[r] = 0.0f;
-#pragma unroll 2
+#pragma unroll 8
             for (int k = 0; k < 64; ++k) {
+                const float4 pn = qb[k + 1 < 64 ? k + 1 : 63];  // next step's j
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
                     const float dx = xj - xi[r];
@@ -270,9 +293,9 @@ __global__ __launch_bounds__(64 * kMutualWaves) void k_mutual(const float4* __re
                         fx += ri;
                     }
                 }
-                xj = rotate_lanes(xj);
-                yj = rotate_lanes(yj);
-                zj = rotate_lanes(zj);
+                xj = pn.x;
+                yj = pn.y;
+                zj = pn.z;
                 fx = rotate_lanes(fx);
                 if constexpr (!POT) {
                     fy = rotate_lanes(fy);
