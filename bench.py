@@ -8,6 +8,11 @@ impact at 1.1 v_escape), FP32.  A step is one force evaluation
 correction) and one move (gi_move_collision) over every particle, with the
 state resident in HBM.
 
+Further legs (each can be switched off): the 64-simulation batched ensemble
+against the same simulations run one after another; the request loop under
+Poisson arrivals; conserved-quantity drift at N = 2,048 (GPU against the
+oracle's float restatement, 1,000 steps) and N = 65,536 (10,000 steps).
+
 Printed: one JSON line.  value = pair interactions per second of whole steps
 (N^2 per step, self pairs included, as the reference kernel visits them).
 roofline: the all-pairs kernel (k_mutual, each pair evaluated once for both
@@ -132,6 +137,95 @@ def ensemble_leg(gi, n_sims: int, steps: int) -> dict:
     }
 
 
+def serving_leg(gi, n_requests: int, mean_gap_s: float, seed: int = 20261206) -> dict:
+    """The request loop under Poisson arrivals: N log-uniform in [2048, 65536]
+    (impactor fraction [0.1, 0.3]), steps uniform in [1000, 20000], impact
+    angle [30, 60] deg, speed [1.0, 1.4] v_esc, target spin [0, 0.6] breakup;
+    exponential inter-arrival gaps of mean mean_gap_s.  Reports time to result
+    (p50 / p99), aggregate throughput and the per-request FP64 energy and
+    angular-momentum changes."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    reqs = []
+    for i in range(n_requests):
+        n = int(round(np.exp(rng.uniform(np.log(2048), np.log(65536)))))
+        ni = max(1, int(n * rng.uniform(0.1, 0.3)))
+        reqs.append(dict(n_target=n - ni, n_impactor=ni, seed=seed + 17 * i, angle_deg=float(rng.uniform(30, 60)),
+                         speed_vesc=float(rng.uniform(1.0, 1.4)), target_spin=float(rng.uniform(0.0, 0.6)),
+                         steps=int(rng.integers(1000, 20001)), arrival_s=t))
+        t += float(rng.exponential(mean_gap_s))
+    res, summ = gi.serve(reqs)
+    ns = np.array([r["n"] for r in res], np.float64)
+    st = np.array([r["steps"] for r in res], np.float64)
+    return {
+        "requests": n_requests,
+        "arrivals": f"Poisson, mean gap {mean_gap_s} s (last arrival {reqs[-1]['arrival_s']:.1f} s)",
+        "n_range": [int(ns.min()), int(ns.max())],
+        "steps_range": [int(st.min()), int(st.max())],
+        "completed": summ["completed"],
+        "wall_s": summ["wall_s"],
+        "sim_steps_per_s": summ["sim_steps_per_s"],
+        "pair_per_s": summ["pair_interactions_per_s"],
+        "p50_time_to_result_s": summ["p50_time_to_result_s"],
+        "p99_time_to_result_s": summ["p99_time_to_result_s"],
+        "max_time_to_result_s": summ["max_time_to_result_s"],
+        "peak_resident_sims": summ["peak_resident_sims"],
+        "peak_resident_particles": summ["peak_resident_particles"],
+        "fp64_energy_rel_change_max": summ["max_energy_rel_change"],
+        "fp64_angmom_rel_change_max": summ["max_angmom_rel_change"],
+    }
+
+
+def drift_leg(gi, steps_small: int, steps_large: int, check_every: int) -> dict:
+    """Conserved quantities over long runs.  N = 2,048 (1,536 + 512, 45 deg,
+    1.2 v_esc): the GPU and the oracle's float restatement of the reference
+    kernels both run steps_small steps from the same state and their FP64
+    E / P / L changes are compared.  N = 65,536 (52,429 + 13,107, target spin
+    0.5 breakup, 45 deg, 1.1 v_esc): steps_large steps on the GPU with FP64
+    E / P / L every check_every steps.  Energy is not conserved by this law
+    (receding contacts use the reduced stiffness KR), angular and linear
+    momentum are."""
+    from oracle import oracle as O
+
+    def rel(c0, c1):
+        L0 = np.linalg.norm(c0[6:9])
+        return {"energy_rel_change": float(abs(c1[2] - c0[2]) / abs(c0[2])),
+                "angmom_rel_change": float(np.linalg.norm(c1[6:9] - c0[6:9]) / L0),
+                "momentum_change_abs": float(np.linalg.norm(c1[3:6] - c0[3:6]))}
+
+    out = {}
+    pos, vel, fc, mc, _ = gi.make_impact(1536, 512, seed=20261202, angle_deg=45.0, speed_vesc=1.2)
+    n = len(pos)
+    c0 = O.conserved_f64(pos, vel, fc)
+    P, V, F = gi.DeviceArray(n, pos), gi.DeviceArray(n, vel), gi.DeviceArray(n, np.zeros((n, 4), np.float32))
+    gi.impact_steps(P, V, F, n, fc, mc, steps_small)
+    gi.sync()
+    g1 = O.conserved_f64(P.download(), V.download(), fc)
+    t0 = time.perf_counter()
+    po, vo, _ = O.impact_steps_f32(pos, vel, fc, mc, steps_small)
+    t_ref = time.perf_counter() - t0
+    r1 = O.conserved_f64(po, vo, fc)
+    out["n2048"] = {"steps": steps_small, "gpu": rel(c0, g1), "reference_float_oracle": rel(c0, r1),
+                    "oracle_seconds": t_ref}
+    pos, vel, fc, mc, _ = gi.make_impact(52429, 13107, seed=20261203, angle_deg=45.0, speed_vesc=1.1,
+                                         target_spin=0.5)
+    n = len(pos)
+    P, V, F = gi.DeviceArray(n, pos), gi.DeviceArray(n, vel), gi.DeviceArray(n, np.zeros((n, 4), np.float32))
+    c0 = gi.conserved_quantities(P, V, n, fc)
+    series = []
+    t0 = time.perf_counter()
+    for k in range(0, steps_large, check_every):
+        gi.impact_steps(P, V, F, n, fc, mc, min(check_every, steps_large - k))
+        c = gi.conserved_quantities(P, V, n, fc)
+        series.append(rel(c0, c))
+    wall = time.perf_counter() - t0
+    out["n65536"] = {"steps": steps_large, "check_every": check_every, "wall_s": wall,
+                     "final": series[-1],
+                     "angmom_rel_change_max": max(x["angmom_rel_change"] for x in series),
+                     "energy_rel_change_series": [x["energy_rel_change"] for x in series]}
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -144,6 +238,11 @@ def main() -> int:
     ap.add_argument("--ensemble-sims", type=int, default=64, help="0 skips the ensemble leg")
     ap.add_argument("--ensemble-steps", type=int, default=100)
     ap.add_argument("--only-ensemble", action="store_true", help="run the ensemble leg alone (profiling)")
+    ap.add_argument("--serve-requests", type=int, default=24,
+                    help="requests of the serving leg (0 = skip)")
+    ap.add_argument("--serve-gap", type=float, default=1.5, help="mean Poisson inter-arrival gap (s)")
+    ap.add_argument("--drift-steps", type=int, default=10000,
+                    help="steps of the N=65,536 conserved-quantity run (0 = skip the drift leg)")
     ap.add_argument("--profile-probe", action="store_true",
                     help="two steps + two 64 MiB copies, for hardware-counter passes")
     args = ap.parse_args()
@@ -288,6 +387,14 @@ def main() -> int:
         log(f"ensemble leg: {args.ensemble_sims} simulations x {args.ensemble_steps} steps")
         result["ensemble"] = ensemble_leg(gi, args.ensemble_sims, args.ensemble_steps)
         log(f"ensemble: {result['ensemble']}")
+    if args.serve_requests > 0:
+        log(f"serving leg: {args.serve_requests} requests, Poisson gaps of mean {args.serve_gap} s")
+        result["serving"] = serving_leg(gi, args.serve_requests, args.serve_gap)
+        log(f"serving: {result['serving']}")
+    if args.drift_steps > 0:
+        log(f"drift leg: N=2048 x 1000 steps against the oracle, N=65536 x {args.drift_steps} steps")
+        result["drift"] = drift_leg(gi, 1000, args.drift_steps, 1000)
+        log(f"drift: {result['drift']}")
     if not args.no_cpu:
         log("timing the CPU baseline")
         result["cpu_baseline"] = cpu_baseline(pos, vel, fc, n, args.cpu_seconds)
