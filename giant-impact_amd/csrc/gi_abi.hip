@@ -5,6 +5,7 @@
 // autotuned table in gi_tuning.h), kernel launches in stream order, HIP-event
 // timing of the all-pairs kernel, and HIP-graph capture of the step loop.
 #include "gi_kernels.hip"
+#include "gi_earth_moon.hip"
 
 #include <cstdarg>
 #include <cstdio>
@@ -138,9 +139,17 @@ struct GraphEntry {
     }
 };
 
+// Scratch of the Earth / Moon finder.
+struct CCWork {
+    DevBuf<int> counts, starts, cellhash, sorted, block_sums, parent, size, mcounts;
+    DevBuf<unsigned long long> best;
+    DevBuf<double> moments;
+};
+
 struct DevCtx {
     int device = -1;
     Workspace ws;
+    CCWork cc;
     DevBuf<double> rc_blocks, rc_sums;
     hipStream_t capture_stream = nullptr;
     hipEvent_t ev_in = nullptr, ev_out = nullptr;
@@ -997,6 +1006,114 @@ int gi_recenter(gi_float4* pos, gi_float4* vel, int n, int nfe, double mass_fe, 
     if (rc) return rc;
     return launch_recenter(ctx, (float4*)pos, (float4*)vel, n, nfe, mass_fe, mass_si, target_com3, target_vel3,
                            as_stream(stream));
+}
+
+int gi_find_earth_moon(const gi_float4* pos, const gi_float4* vel, int n, int nfe1, int nfe2, int nsi1,
+                       double mass_fe, double mass_si, float touch, int* group, gi_earth_moon* out,
+                       gi_stream_t stream) {
+    if (n <= 0 || !pos || !vel || !out || nfe1 < 0 || nfe2 < 0 || nsi1 < 0 || (long long)nfe1 + nfe2 + nsi1 > n ||
+        !(touch > 0.0f) || !(touch < 1e30f))
+        return set_error(GI_E_INVALID_ARG, "gi_find_earth_moon: invalid arguments");
+    DevCtx* ctx;
+    int rc = current_ctx(&ctx);
+    if (rc) return rc;
+    CCWork& w = ctx->cc;
+    hipStream_t st = as_stream(stream);
+    unsigned m = 64;
+    while (m < (unsigned)n) m <<= 1;
+    if ((m + 1023) / 1024 > 4096) return set_error(GI_E_TOO_LARGE, "gi_find_earth_moon: n too large (%d)", n);
+    const int nbs = nblk(m, 1024);
+    if ((rc = w.counts.ensure(m + 1)) || (rc = w.starts.ensure(m + 2)) || (rc = w.cellhash.ensure(n)) ||
+        (rc = w.sorted.ensure(n)) || (rc = w.block_sums.ensure(nbs + 8)) || (rc = w.parent.ensure(n)) ||
+        (rc = w.size.ensure(n + 1)) || (rc = w.best.ensure(2)))
+        return rc;
+    // cells a sixteenth larger than touch: a touching pair is never two cells apart
+    const double inv_c = 1.0 / ((double)touch * 1.0625);
+    const float4* P = (const float4*)pos;
+    hipLaunchKernelGGL(k_fill_int, dim3(nblk(m, 256)), dim3(256), 0, st, w.counts.p, (int)m, 0);
+    hipLaunchKernelGGL(k_cc_count, dim3(nblk(n, 256)), dim3(256), 0, st, P, n, inv_c, m - 1, w.counts.p,
+                       w.cellhash.p);
+    hipLaunchKernelGGL(k_scan_blocks, dim3(nbs), dim3(256), 0, st, w.counts.p, (int)m, w.starts.p, w.block_sums.p);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, st, w.block_sums.p, nbs, w.starts.p + m);
+    hipLaunchKernelGGL(k_scan_add, dim3(nblk(m, 256)), dim3(256), 0, st, w.starts.p, (int)m, w.block_sums.p);
+    hipLaunchKernelGGL(k_grid_scatter, dim3(nblk(n, 256)), dim3(256), 0, st, n, w.cellhash.p, w.starts.p,
+                       w.counts.p, w.sorted.p);
+    hipLaunchKernelGGL(k_cc_init, dim3(nblk(n, 256)), dim3(256), 0, st, w.parent.p, w.size.p, n);
+    hipLaunchKernelGGL(k_cc_union, dim3(nblk(n, 256)), dim3(256), 0, st, P, n, inv_c, m - 1, touch, w.starts.p,
+                       w.sorted.p, w.parent.p);
+    hipLaunchKernelGGL(k_cc_compress, dim3(nblk(n, 256)), dim3(256), 0, st, w.parent.p, w.size.p, n);
+    const unsigned long long zero[2] = {0ull, 0ull};
+    GI_HIP(hipMemcpyAsync(w.best.p, zero, sizeof zero, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_cc_top, dim3(nblk(n, 256)), dim3(256), 0, st, w.size.p, n, -1, w.best.p);
+    GI_HIP(hipGetLastError());
+    unsigned long long best[2];
+    GI_HIP(hipMemcpyAsync(best, w.best.p, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+    GI_HIP(hipStreamSynchronize(st));
+    const int earth = (int)~(unsigned)(best[0] & 0xffffffffull);
+    hipLaunchKernelGGL(k_cc_top, dim3(nblk(n, 256)), dim3(256), 0, st, w.size.p, n, earth, w.best.p + 1);
+    GI_HIP(hipGetLastError());
+    GI_HIP(hipMemcpyAsync(best + 1, w.best.p + 1, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+    GI_HIP(hipStreamSynchronize(st));
+    const int moon = best[1] ? (int)~(unsigned)(best[1] & 0xffffffffull) : -1;
+    // moments of the system, the Earth and the Moon
+    const int nb = std::min(1024, nblk(n, kCCBlock));
+    if ((rc = w.moments.ensure((size_t)nb * kCCClasses * kCCMoments)) || (rc = w.mcounts.ensure((size_t)nb * kCCCounts)))
+        return rc;
+    hipLaunchKernelGGL(k_cc_moments, dim3(nb), dim3(kCCBlock), 0, st, P, (const float4*)vel, n, w.parent.p, earth,
+                       moon, nfe1, nfe2, nsi1, mass_fe, mass_si, w.moments.p, w.mcounts.p);
+    GI_HIP(hipGetLastError());
+    std::vector<double> mom((size_t)nb * kCCClasses * kCCMoments);
+    std::vector<int> cnt((size_t)nb * kCCCounts);
+    std::vector<int> sizes(n);
+    GI_HIP(hipMemcpyAsync(mom.data(), w.moments.p, mom.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    GI_HIP(hipMemcpyAsync(cnt.data(), w.mcounts.p, cnt.size() * sizeof(int), hipMemcpyDeviceToHost, st));
+    GI_HIP(hipMemcpyAsync(sizes.data(), w.size.p, sizes.size() * sizeof(int), hipMemcpyDeviceToHost, st));
+    if (group) GI_HIP(hipMemcpyAsync(group, w.parent.p, (size_t)n * sizeof(int), hipMemcpyDeviceToDevice, st));
+    GI_HIP(hipStreamSynchronize(st));
+    // sums over blocks in block order; scope 1 (Earth-Moon) = Earth + Moon
+    double M[4][kCCMoments] = {};
+    for (int b = 0; b < nb; ++b)
+        for (int c = 0; c < kCCClasses; ++c)
+            for (int k = 0; k < kCCMoments; ++k) M[c == 0 ? 0 : c + 1][k] += mom[((size_t)b * kCCClasses + c) * kCCMoments + k];
+    for (int k = 0; k < kCCMoments; ++k) M[1][k] = M[2][k] + M[3][k];
+    std::memset(out, 0, sizeof *out);
+    int c8[kCCCounts] = {};
+    for (int b = 0; b < nb; ++b)
+        for (int k = 0; k < kCCCounts; ++k) c8[k] += cnt[(size_t)b * kCCCounts + k];
+    out->n_groups = 0;
+    for (int i = 0; i < n; ++i) out->n_groups += sizes[i] > 0;
+    out->earth_id = earth;
+    out->moon_id = moon;
+    out->earth_count = sizes[earth];
+    out->moon_count = moon >= 0 ? sizes[moon] : 0;
+    out->earth_fe1 = c8[0];
+    out->earth_fe2 = c8[1];
+    out->earth_si1 = c8[2];
+    out->earth_si2 = c8[3];
+    out->moon_fe1 = c8[4];
+    out->moon_fe2 = c8[5];
+    out->moon_si1 = c8[6];
+    out->moon_si2 = c8[7];
+    for (int s = 0; s < 4; ++s) {
+        const double* a = M[s];
+        const double mass = a[0];
+        out->mass[s] = mass;
+        for (int d = 0; d < 3; ++d) {
+            out->com[s][d] = a[1 + d] / mass;  // 0/0 = NaN for an empty scope, as the reference divides
+            out->vel[s][d] = a[4 + d] / mass;
+        }
+        // L about the scope's centre of mass relative to its mean velocity:
+        // sum m x cross v - (sum m x) cross (sum m v) / M
+        const double* X = a + 1;
+        const double* Pm = a + 4;
+        const double cx = (X[1] * Pm[2] - X[2] * Pm[1]) / mass;
+        const double cy = (X[2] * Pm[0] - X[0] * Pm[2]) / mass;
+        const double cz = (X[0] * Pm[1] - X[1] * Pm[0]) / mass;
+        out->angmom[s][0] = a[7] - cx;
+        out->angmom[s][1] = a[8] - cy;
+        out->angmom[s][2] = a[9] - cz;
+    }
+    return 0;
 }
 
 int gi_conserved_quantities(const gi_float4* pos, const gi_float4* vel, int n, const gi_force_collision_constants* c,

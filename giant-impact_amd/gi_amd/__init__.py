@@ -184,6 +184,14 @@ class ImpactInfo(C.Structure):
 _lib = None
 
 
+class EarthMoon(C.Structure):  # gi_earth_moon
+    _fields_ = [(k, C.c_int) for k in ("n_groups", "earth_id", "moon_id", "earth_count", "moon_count",
+                                          "earth_fe1", "earth_fe2", "earth_si1", "earth_si2",
+                                          "moon_fe1", "moon_fe2", "moon_si1", "moon_si2")] + [
+        ("mass", C.c_double * 4), ("com", (C.c_double * 3) * 4), ("vel", (C.c_double * 3) * 4),
+        ("angmom", (C.c_double * 3) * 4)]
+
+
 def lib_path() -> Path:
     return LIB_PATH
 
@@ -260,6 +268,8 @@ def load() -> C.CDLL:
         "gi_set_tiling": (i, [i, i, C.c_longlong, C.c_longlong, C.c_longlong]),
         "gi_plan_info": (i, [P(i), P(i), P(C.c_longlong), P(C.c_longlong)]),
         "gi_tiling_for": (i, [C.c_longlong, P(i), P(i), P(C.c_longlong), P(C.c_longlong), P(C.c_longlong)]),
+        "gi_find_earth_moon": (i, [vp, vp, i, i, i, i, C.c_double, C.c_double, C.c_float, vp, P(EarthMoon), vp]),
+        "gi_write_run_stats": (i, [C.c_char_p, P(EarthMoon), P(Units), C.c_double]),
         "gi_set_batched_tiling": (i, [i, C.c_longlong, i]),
         "gi_serve_run": (i, [vp, i, vp, vp, vp]),
         "gi_spin_body": (i, [vp, vp, i, i, i, C.c_double, C.c_double, C.c_double, vp]),
@@ -431,6 +441,23 @@ def force_separate(pos, vel, force, n: int, fc: ForceSeparateConstants, stream=N
 
 def move_damped_separate(pos, vel, force, n: int, mc: MoveSeparateConstants, d1: float, d2: float, stream=None):
     call("gi_move_damped_separate", _p(pos), _p(vel), _p(force), int(n), C.byref(mc), float(d1), float(d2), _p(stream))
+
+
+def find_earth_moon(P, V, n: int, nfe1: int, nfe2: int, nsi1: int, mass_fe: float, mass_si: float = 1.0,
+                    touch: float = 1.5, with_groups: bool = False, stream=None) -> dict:
+    """The Earth / Moon finder on device arrays; scopes 0 system, 1 Earth-Moon,
+    2 Earth, 3 Moon."""
+    em = EarthMoon()
+    G = DeviceArray(max(n, 1) // 4 + 1) if with_groups else None  # n ints = n/4 float4 records
+    call("gi_find_earth_moon", _p(P), _p(V), int(n), int(nfe1), int(nfe2), int(nsi1), float(mass_fe), float(mass_si),
+         float(touch), _p(G), C.byref(em), _p(stream))
+    out = {k: getattr(em, k) for k, _ in EarthMoon._fields_[:13]}
+    out["mass"] = np.array(em.mass[:])
+    for k in ("com", "vel", "angmom"):
+        out[k] = np.array([list(r) for r in getattr(em, k)])
+    if with_groups:
+        out["group"] = G.download().view(np.int32).reshape(-1)[:n].copy()
+    return out
 
 
 def tiling_for(n: int) -> dict:

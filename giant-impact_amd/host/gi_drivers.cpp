@@ -74,9 +74,97 @@ int write_centering(const std::string &path, const float c[6]) {
     return std::fclose(f) == 0 ? 0 : GI_E_IO;
 }
 
+// The reference's "angle off the ecliptic plane" of a vector (the ecliptic is
+// the xz-plane; impactBasicFunctions.h:976-980).
+double off_ecliptic_deg(double x, double y, double z) {
+    const double size = std::sqrt(x * x + y * y + z * z) * std::sqrt(x * x + z * z);
+    const double angle = std::acos((x * x + z * z) / size);
+    return 90.0 - angle * 180.0 / kPi;
+}
+
 }  // namespace
 
 extern "C" {
+
+int gi_write_run_stats(const char *path, const gi_earth_moon *em, const gi_units *u, double time) {
+    if (!path || !em || !u) return GI_E_INVALID_ARG;
+    FILE *f = std::fopen(path, "wb");
+    if (!f) return GI_E_IO;
+    // Text, order and unit conversions of recordFinalImpactStat
+    // (impactBasicFunctions.h:883-1072).  Scopes: 0 system, 1 Earth-Moon,
+    // 2 Earth, 3 Moon.
+    const double time_c = u->unit_time, length_c = u->unit_length, mass_c = u->unit_mass;
+    const double velocity_c = u->unit_length / u->unit_time;
+    const double momentum_c = u->unit_mass * u->unit_length * u->unit_length / u->unit_time;
+    std::fprintf(f, "\n\n\n*************************************************************************\n\n");
+    std::fprintf(f, "\nThe following are the final stats of the run when time = %f hours\n", time * time_c / 3600.0);
+    std::fprintf(f, "\nDistance is measured in Kilometers");
+    std::fprintf(f, "\nMass is measured in Kilograms");
+    std::fprintf(f, "\nTime is measured in seconds");
+    std::fprintf(f, "\nVelocity is measured in Kilometers/second");
+    std::fprintf(f, "\nAngular momentun is measured in Kilograms*Kilometers*Kilometers/seconds\n");
+    std::fprintf(f, "\nThe mass of Earth \t\t= %e", em->mass[2] * mass_c);
+    std::fprintf(f, "\nThe mass of Moon \t\t= %e", em->mass[3] * mass_c);
+    if (em->mass[3] != 0.0) std::fprintf(f, "\nThe mass ratio Earth/Moon \t= %f\n", em->mass[2] / em->mass[3]);
+    std::fprintf(f, "\nMoon iron from body 1 \t\t= %d", em->moon_fe1);
+    std::fprintf(f, "\nMoon silicate from body 1 \t= %d", em->moon_si1);
+    std::fprintf(f, "\nMoon iron from body 2 \t\t= %d", em->moon_fe2);
+    std::fprintf(f, "\nMoon silicate from body 2 \t= %d", em->moon_si2);
+    if (em->moon_fe2 + em->moon_si2 == 0)
+        std::fprintf(f, "\nThe Moon is only composed of elements from body 1\n");
+    else if (em->moon_fe1 + em->moon_si1 == 0)
+        std::fprintf(f, "\nThe Moon is only composed of elements from body 2\n");
+    else
+        std::fprintf(f, "\nMoon ratio body1/body2 \t\t= %f\n",
+                     (float)(em->moon_fe1 + em->moon_si1) / (float)(em->moon_fe2 + em->moon_si2));
+    std::fprintf(f, "\nEarth iron from body 1 \t\t= %d", em->earth_fe1);
+    std::fprintf(f, "\nEarth silicate from body 1 \t= %d", em->earth_si1);
+    std::fprintf(f, "\nEarth iron from body 2 \t\t= %d", em->earth_fe2);
+    std::fprintf(f, "\nEarth silicate from body 2 \t= %d", em->earth_si2);
+    if (em->earth_fe2 + em->earth_si2 == 0)
+        std::fprintf(f, "\nThe Earth is only composed of elements from body 1\n");
+    else if (em->earth_fe1 + em->earth_si1 == 0)
+        std::fprintf(f, "\nThe Earth is only composed of elements from body 2\n");
+    else
+        std::fprintf(f, "\nEarth ratio body1/body2 \t\t= %f\n",
+                     (float)(em->earth_fe1 + em->earth_si1) / (float)(em->earth_fe2 + em->earth_si2));
+    static const char *lname[3] = {"the Earth Moon system \t\t", "the Earth \t\t\t\t", "the Moon \t\t\t\t"};
+    static const char *mname[3] = {"the system \t", "the Earth \t\t", "the Moon \t\t"};
+    static const char *aname[3] = {"the system's", "the Earth's", "the Moon's"};
+    for (int k = 0; k < 3; ++k) {
+        const double x = em->angmom[k + 1][0] * momentum_c, y = em->angmom[k + 1][1] * momentum_c,
+                     z = em->angmom[k + 1][2] * momentum_c;
+        std::fprintf(f, "\nAngular momentum of %s= (%e, %e, %e)", lname[k], x, y, z);
+        std::fprintf(f, "\nMagnitude of the angular momentum of %s= %e", mname[k], std::sqrt(x * x + y * y + z * z));
+        std::fprintf(f, "\nAngle off ecliptic plane of %s rotation \t= %f\n", aname[k], off_ecliptic_deg(x, y, z));
+    }
+    static const char *cname[3] = {"the Earth-Moon system \t\t", "the Earth system \t\t\t",
+                                   "the Moon system \t\t\t"};
+    for (int k = 0; k < 3; ++k)
+        std::fprintf(f, "\nCenter of mass of %s= (%f, %f, %f)%s", cname[k], em->com[k + 1][0] * length_c,
+                     em->com[k + 1][1] * length_c, em->com[k + 1][2] * length_c, k == 2 ? "\n" : "");
+    for (int k = 0; k < 3; ++k)
+        std::fprintf(f, "\nLinear Velocity of %s= (%f, %f, %f)%s", cname[k], em->vel[k + 1][0] * velocity_c,
+                     em->vel[k + 1][1] * velocity_c, em->vel[k + 1][2] * velocity_c, k == 2 ? "\n" : "");
+    std::fprintf(f, "\n*****Stats of the entire system to check the numerical scheme's validity*****\n");
+    std::fprintf(f, "\nCenter of mass of the entire system \t\t        = (%f, %f, %f)\n", em->com[0][0] * length_c,
+                 em->com[0][1] * length_c, em->com[0][2] * length_c);
+    std::fprintf(f, "\nTotal mass of the entire system \t= %f\n", em->mass[0]);
+    {
+        const double x = em->vel[0][0] * velocity_c, y = em->vel[0][1] * velocity_c, z = em->vel[0][2] * velocity_c;
+        std::fprintf(f, "\nLinear velocity of the entire system system \t\t= (%f, %f, %f)", x, y, z);
+        std::fprintf(f, "\nMagnitude of the linear velocity of the entire system \t= %f\n", std::sqrt(x * x + y * y + z * z));
+    }
+    {
+        const double x = em->angmom[0][0] * momentum_c, y = em->angmom[0][1] * momentum_c,
+                     z = em->angmom[0][2] * momentum_c;
+        std::fprintf(f, "\nAngular momentum of the entire system system \t\t= (%e, %e, %e)", x, y, z);
+        std::fprintf(f, "\nMagnitude of the angular momentum of the entire system \t= %e\n", std::sqrt(x * x + y * y + z * z));
+    }
+    std::fprintf(f, "\n\n*************************************************************************\n\n\n");
+    return std::fclose(f) == 0 ? 0 : GI_E_IO;
+}
+
 
 int gi_spin_body(const gi_float4 *pos, gi_float4 *vel, int start, int stop, int stop_fe, double mass_fe,
                  double mass_si, double total_mass, const float *spin4) {
@@ -428,6 +516,13 @@ int gi_impact_run(const char *param_dir, const char *setup_path, const char *run
     if (std::fclose(ff) != 0 && !rc) rc = GI_E_IO;
     if (rc) return rc;
     if ((rc = write_centering(join(run_dir, "viewCenteringInfo"), centering))) return rc;
+    // The final statistics of a finished run (impactAddTime.cu:457-465): the
+    // Earth / Moon finder on the device state, then runStatsFile.
+    gi_earth_moon em;
+    if ((rc = gi_find_earth_moon(P, V, n, u.nfe1, u.nfe2, u.nsi1, u.mass_fe, u.mass_si, (float)(u.diameter * 1.5),
+                                 nullptr, &em, nullptr)))
+        return rc;
+    if ((rc = gi_write_run_stats(join(run_dir, "runStatsFile").c_str(), &em, &u, run_time))) return rc;
     if (info) {
         info->n = n;
         info->steps = n_steps;

@@ -249,6 +249,36 @@ int gi_ensemble_conserved(gi_ensemble *e, double *out);
 /* Total particles and total pair interactions of one step of the active set. */
 int gi_ensemble_work(gi_ensemble *e, long long *particles, double *pairs);
 
+/* ---- final analysis: the Earth / Moon finder ------------------------------ */
+
+/* findEarthAndMoon and the final statistics' sums (recordFinalImpactStat,
+ * Code/ImpactCode/impactBasicFunctions.h:109-217,219-652,883-1072).  Scopes:
+ * 0 entire system, 1 Earth-Moon system, 2 Earth, 3 Moon, as the reference's
+ * getMassCollision(scope) numbers them.  Sums are FP64 (the reference sums
+ * in float); an empty scope has mass 0 and NaN centre and velocity, as the
+ * reference's division by a zero mass gives. */
+typedef struct gi_earth_moon {
+    int n_groups;           /* groups of elements closer than touch (connected) */
+    int earth_id, moon_id;  /* group numbers: the smallest element index of each; moon_id -1 when absent */
+    int earth_count, moon_count;
+    int earth_fe1, earth_fe2, earth_si1, earth_si2; /* elements by body and material */
+    int moon_fe1, moon_fe2, moon_si1, moon_si2;
+    double mass[4];
+    double com[4][3];
+    double vel[4][3];       /* mass-weighted mean velocity */
+    double angmom[4][3];    /* about the scope's centre of mass, relative to its mean velocity */
+} gi_earth_moon;
+
+/* Groups of pos[n] (device, impact layout [Fe1, Fe2, Si1, Si2] with counts
+ * nfe1, nfe2, nsi1; the rest is silicate of body 2) joined by pairs with the
+ * reference's float test sqrt(dx*dx + dy*dy + dz*dz) < touch (the reference
+ * uses touch = 1.5 * Diameter); the Earth is the largest group and the Moon
+ * the next, ties to the smaller group number.  group (optional device int[n])
+ * receives each element's group number.  Blocking. */
+int gi_find_earth_moon(const gi_float4 *pos, const gi_float4 *vel, int n, int nfe1, int nfe2, int nsi1,
+                       double mass_fe, double mass_si, float touch, int *group, gi_earth_moon *out,
+                       gi_stream_t stream);
+
 /* ---- tiling of the all-pairs pass (the autotuner's control) -------------- */
 
 /* Override the tiling chosen from the tuning table (tuning/gfx950.txt beside

@@ -157,9 +157,18 @@ typedef struct gi_impact_run_info {
  * param_dir, setupImpactAddTime from setup_path, viewCenteringInfo and the
  * last frame of impactPosVel from run_dir; advance AdditionalTime days,
  * appending a frame every RecordRate steps and recentring every RecenterRate
- * steps (FP64 on the device); rewrite viewCenteringInfo at the end. */
+ * steps (FP64 on the device); rewrite viewCenteringInfo and write
+ * runStatsFile (the Earth / Moon finder on the device) at the end. */
 int gi_impact_run(const char *param_dir, const char *setup_path, const char *run_dir,
                   gi_impact_run_info *info);
+
+/* Write runStatsFile as recordFinalImpactStat does
+ * (Code/ImpactCode/impactBasicFunctions.h:883-1072): the Earth and Moon
+ * masses, composition by body and material, angular momenta with their angle
+ * off the ecliptic (xz) plane, centres and velocities, then the whole system,
+ * in km, kg and s through the units of u.  time is the simulation time
+ * (simulation units).  gi_impact_run writes it at the end of every run. */
+int gi_write_run_stats(const char *path, const gi_earth_moon *em, const gi_units *u, double time);
 
 /* ---- request loop (continuous batching of impact-sweep jobs) ------------- */
 

@@ -747,3 +747,190 @@ void or_impact_steps_f32(or_float4 *pos, or_float4 *vel, or_float4 *force, int n
         }
     }
 }
+
+/* ---- final analysis --------------------------------------------------------- */
+
+/* findEarthAndMoon, Code/ImpactCode/impactBasicFunctions.h:109-217.  Groups
+ * are flooded from the first unassigned element; the group counter advances
+ * for every i (:150), so a group's number is the index of its first element.
+ * The touch test is float: mag = sqrt(dx*dx + dy*dy + dz*dz) < touch
+ * (:136-140).  Earth = the largest group, Moon = the next, ties to the
+ * smaller number (strict > in :170-187).  group[n] receives the numbers;
+ * returns the number of non-empty groups. */
+int or_find_earth_moon(const or_float4 *pos, int n, float touch, int *group, int *earth_id, int *moon_id,
+                       int *earth_count, int *moon_count)
+{
+    int *used = (int *)calloc((size_t)n, sizeof(int));
+    int *size = (int *)calloc((size_t)n, sizeof(int));
+    for (int i = 0; i < n; i++) group[i] = -1;
+    int group_number = 0;
+    for (int i = 0; i < n; i++) {
+        if (group[i] == -1) {
+            group[i] = group_number;
+            int k = i;
+            while (k < n) {
+                if (group[k] == group_number && used[k] == 0) {
+                    for (int j = i; j < n; j++) {
+                        float dx = pos[k].x - pos[j].x;
+                        float dy = pos[k].y - pos[j].y;
+                        float dz = pos[k].z - pos[j].z;
+                        float mag = sqrtf(dx * dx + dy * dy + dz * dz);
+                        if (mag < touch) group[j] = group_number;
+                    }
+                    used[k] = 1;
+                    k = i;
+                } else {
+                    k++;
+                }
+            }
+        }
+        group_number++;
+    }
+    for (int j = 0; j < n; j++) size[group[j]]++;
+    int e = -1, ne = 0, m = -1, nm = 0, groups = 0;
+    for (int g = 0; g < n; g++) {
+        if (size[g] > 0) groups++;
+        if (size[g] > ne) {
+            ne = size[g];
+            e = g;
+        }
+    }
+    for (int g = 0; g < n; g++)
+        if (size[g] > nm && g != e) {
+            nm = size[g];
+            m = g;
+        }
+    *earth_id = e;
+    *moon_id = m;
+    *earth_count = ne;
+    *moon_count = nm;
+    free(used);
+    free(size);
+    return groups;
+}
+
+/* Members of scope s in the reference's order: 0 every element, 1 the Earth's
+ * then the Moon's (EarthIndex then MoonIndex, :198-212), 2 the Earth's, 3 the
+ * Moon's. */
+static int em_members(const int *group, int n, int earth, int moon, int scope, int *idx)
+{
+    int c = 0;
+    if (scope == 0) {
+        for (int i = 0; i < n; i++) idx[c++] = i;
+        return c;
+    }
+    if (scope == 1 || scope == 2)
+        for (int i = 0; i < n; i++)
+            if (group[i] == earth) idx[c++] = i;
+    if (scope == 1 || scope == 3)
+        for (int i = 0; i < n; i++)
+            if (moon >= 0 && group[i] == moon) idx[c++] = i;
+    return c;
+}
+
+/* getMassCollision / getCenterOfMassCollision / getLinearVelocityCollision /
+ * getAngularMomentumCollision for scopes 0..3 (impactBasicFunctions.h:219-652),
+ * float accumulators with the double mass products as written there.
+ * Outputs: mass4[4], com12[4][3], vel12[4][3], L12[4][3]. */
+void or_final_stats_f32(const or_float4 *pos, const or_float4 *vel, int n, int nfe, const int *group,
+                        int earth, int moon, double mass_fe, double mass_si, float *mass4, float *com12,
+                        float *vel12, float *L12)
+{
+    int *idx = (int *)malloc((size_t)n * sizeof(int));
+    for (int s = 0; s < 4; s++) {
+        int c = em_members(group, n, earth, moon, s, idx);
+        float mass = 0.0f;
+        for (int a = 0; a < c; a++) mass = (float)((double)mass + (idx[a] < nfe ? mass_fe : mass_si));
+        float cx = 0, cy = 0, cz = 0, vx = 0, vy = 0, vz = 0;
+        for (int a = 0; a < c; a++) {
+            int i = idx[a];
+            double m = i < nfe ? mass_fe : mass_si;
+            cx = (float)((double)cx + (double)pos[i].x * m);
+            cy = (float)((double)cy + (double)pos[i].y * m);
+            cz = (float)((double)cz + (double)pos[i].z * m);
+        }
+        cx /= mass;
+        cy /= mass;
+        cz /= mass;
+        for (int a = 0; a < c; a++) {
+            int i = idx[a];
+            double m = i < nfe ? mass_fe : mass_si;
+            vx = (float)((double)vx + (double)vel[i].x * m);
+            vy = (float)((double)vy + (double)vel[i].y * m);
+            vz = (float)((double)vz + (double)vel[i].z * m);
+        }
+        vx /= mass;
+        vy /= mass;
+        vz /= mass;
+        float lx = 0, ly = 0, lz = 0;
+        for (int a = 0; a < c; a++) {
+            int i = idx[a];
+            double m = i < nfe ? mass_fe : mass_si;
+            float rx = pos[i].x - cx, ry = pos[i].y - cy, rz = pos[i].z - cz;
+            float ux = vel[i].x - vx, uy = vel[i].y - vy, uz = vel[i].z - vz;
+            lx = (float)((double)lx + (double)(ry * uz - rz * uy) * m);
+            ly = (float)((double)ly + (double)(-(rx * uz - rz * ux)) * m);
+            lz = (float)((double)lz + (double)(rx * uy - ry * ux) * m);
+        }
+        mass4[s] = mass;
+        com12[3 * s + 0] = cx;
+        com12[3 * s + 1] = cy;
+        com12[3 * s + 2] = cz;
+        vel12[3 * s + 0] = vx;
+        vel12[3 * s + 1] = vy;
+        vel12[3 * s + 2] = vz;
+        L12[3 * s + 0] = lx;
+        L12[3 * s + 1] = ly;
+        L12[3 * s + 2] = lz;
+    }
+    free(idx);
+}
+
+/* The same sums in FP64 (truth for the product's FP64 reductions). */
+void or_final_stats_f64(const or_float4 *pos, const or_float4 *vel, int n, int nfe, const int *group,
+                        int earth, int moon, double mass_fe, double mass_si, double *mass4, double *com12,
+                        double *vel12, double *L12)
+{
+    int *idx = (int *)malloc((size_t)n * sizeof(int));
+    for (int s = 0; s < 4; s++) {
+        int c = em_members(group, n, earth, moon, s, idx);
+        double mass = 0, cx = 0, cy = 0, cz = 0, vx = 0, vy = 0, vz = 0, lx = 0, ly = 0, lz = 0;
+        for (int a = 0; a < c; a++) {
+            int i = idx[a];
+            double m = i < nfe ? mass_fe : mass_si;
+            mass += m;
+            cx += m * pos[i].x;
+            cy += m * pos[i].y;
+            cz += m * pos[i].z;
+            vx += m * vel[i].x;
+            vy += m * vel[i].y;
+            vz += m * vel[i].z;
+        }
+        cx /= mass;
+        cy /= mass;
+        cz /= mass;
+        vx /= mass;
+        vy /= mass;
+        vz /= mass;
+        for (int a = 0; a < c; a++) {
+            int i = idx[a];
+            double m = i < nfe ? mass_fe : mass_si;
+            double rx = pos[i].x - cx, ry = pos[i].y - cy, rz = pos[i].z - cz;
+            double ux = vel[i].x - vx, uy = vel[i].y - vy, uz = vel[i].z - vz;
+            lx += m * (ry * uz - rz * uy);
+            ly += m * (rz * ux - rx * uz);
+            lz += m * (rx * uy - ry * ux);
+        }
+        mass4[s] = mass;
+        com12[3 * s + 0] = cx;
+        com12[3 * s + 1] = cy;
+        com12[3 * s + 2] = cz;
+        vel12[3 * s + 0] = vx;
+        vel12[3 * s + 1] = vy;
+        vel12[3 * s + 2] = vz;
+        L12[3 * s + 0] = lx;
+        L12[3 * s + 1] = ly;
+        L12[3 * s + 2] = lz;
+    }
+    free(idx);
+}

@@ -120,6 +120,17 @@ void or_body_vel(const or_float4 *vel, int start_fe, int stop_fe, int stop_si, d
  * revolutions per time unit w. Returns 0, or -1 for a zero direction. */
 int or_spin_body(const or_float4 *pos, or_float4 *vel, int start, int stop, int stop_fe,
                  double mass_fe, double mass_si, double total_mass, const float *spin4);
+/* findEarthAndMoon (impactBasicFunctions.h:109-217); returns the number of groups. */
+int or_find_earth_moon(const or_float4 *pos, int n, float touch, int *group, int *earth_id, int *moon_id,
+                       int *earth_count, int *moon_count);
+/* get{Mass,CenterOfMass,LinearVelocity,AngularMomentum}Collision(scope 0..3)
+ * (impactBasicFunctions.h:219-652) in float, and an FP64 twin. */
+void or_final_stats_f32(const or_float4 *pos, const or_float4 *vel, int n, int nfe, const int *group,
+                        int earth, int moon, double mass_fe, double mass_si, float *mass4, float *com12,
+                        float *vel12, float *L12);
+void or_final_stats_f64(const or_float4 *pos, const or_float4 *vel, int n, int nfe, const int *group,
+                        int earth, int moon, double mass_fe, double mass_si, double *mass4, double *com12,
+                        double *vel12, double *L12);
 /* One step of the nBodyCollisionSingleGPU loop (force, move) repeated n_steps times,
  * with the recenter every recenter_rate steps (0 = never). */
 void or_impact_steps_f32(or_float4 *pos, or_float4 *vel, or_float4 *force, int n,

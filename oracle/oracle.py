@@ -90,6 +90,9 @@ def load() -> C.CDLL:
         "or_force_collision_f64": (None, [vp, vp, vp, vp, vp, d, i, P(FC), i, i, i]),
         "or_force_separate_f64": (None, [vp, vp, vp, vp, vp, d, i, P(FS), i]),
         "or_conserved_f64": (None, [vp, vp, i, P(FC), vp, i]),
+        "or_find_earth_moon": (i, [vp, i, C.c_float, vp, P(i), P(i), P(i), P(i)]),
+        "or_final_stats_f32": (None, [vp, vp, i, i, vp, i, i, C.c_double, C.c_double, vp, vp, vp, vp]),
+        "or_final_stats_f64": (None, [vp, vp, i, i, vp, i, i, C.c_double, C.c_double, vp, vp, vp, vp]),
         "or_com_vel_collision_f32": (None, [vp, vp, i, i, d, d, vp]),
         "or_recenter_f32": (None, [vp, vp, i, i, d, d, vp, vp]),
         "or_body_com": (None, [vp, i, i, i, d, d, d, vp]),
@@ -204,6 +207,31 @@ def force_separate_f64(pos, vel, fs, margin=1e-6, nthreads=None):
         _ptr(pos), _ptr(vel), _ptr(f), _ptr(cond), _ptr(near), margin, n, C.byref(as_fs(fs)), int(nthreads or threads())
     )
     return f, cond, near.astype(bool)
+
+
+def find_earth_moon(pos, touch: float = 1.5) -> dict:
+    """findEarthAndMoon restated: group numbers (index of each group's first
+    element), the Earth and Moon group numbers and sizes."""
+    pos = _f4(pos)
+    n = pos.shape[0]
+    group = np.zeros(n, dtype=np.int32)
+    e, m, ne, nm = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+    groups = load().or_find_earth_moon(_ptr(pos), n, float(touch), _ptr(group), C.byref(e), C.byref(m), C.byref(ne),
+                                       C.byref(nm))
+    return {"group": group, "groups": groups, "earth_id": e.value, "moon_id": m.value, "earth_count": ne.value,
+            "moon_count": nm.value}
+
+
+def final_stats(pos, vel, nfe, group, earth, moon, mass_fe, mass_si=1.0, fp64=False) -> dict:
+    """get*Collision(scope 0..3) restated (float, or the FP64 twin)."""
+    pos, vel = _f4(pos), _f4(vel)
+    group = np.ascontiguousarray(group, dtype=np.int32)
+    dt = np.float64 if fp64 else np.float32
+    mass, com, v, L = np.zeros(4, dt), np.zeros((4, 3), dt), np.zeros((4, 3), dt), np.zeros((4, 3), dt)
+    f = load().or_final_stats_f64 if fp64 else load().or_final_stats_f32
+    f(_ptr(pos), _ptr(vel), pos.shape[0], int(nfe), _ptr(group), int(earth), int(moon), float(mass_fe), float(mass_si),
+      _ptr(mass), _ptr(com), _ptr(v), _ptr(L))
+    return {"mass": mass, "com": com, "vel": v, "angmom": L}
 
 
 def conserved_f64(pos, vel, fc, nthreads=None) -> np.ndarray:

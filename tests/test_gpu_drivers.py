@@ -98,6 +98,18 @@ def test_impact_pipeline_files_resume_and_recenter(gpu, tmp_path):
     assert info1.frames_written == info1.steps // 100
     f1 = gpu.read_frames(run / "impactPosVel", n)
     assert len(f1) == 1 + info1.frames_written
+    # runStatsFile of the finished run: the finder's counts on the final state
+    stats = (run / "runStatsFile").read_text()
+    vals = {}
+    for line in stats.splitlines():
+        if "=" in line:
+            k, v = line.split("=", 1)
+            vals[" ".join(k.split())] = v.strip()
+    counts = [int(vals[f"{w} {m} from body {b}"]) for w in ("Earth", "Moon") for b in (1, 2)
+              for m in ("iron", "silicate")]
+    assert sum(counts) <= n and counts[0] + counts[1] > 0
+    assert float(vals["The mass of Earth"]) > 0.0
+    assert "final stats of the run when time" in stats
     times = [f[0] for f in f1]
     assert all(b > a for a, b in zip(times, times[1:]))
     # resume: the second run starts from the last frame

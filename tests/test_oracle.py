@@ -364,3 +364,105 @@ def test_spin_body_is_rigid_rotation(oracle):
     assert np.allclose(v[:, :3], expect, rtol=1e-5, atol=1e-6)
     with pytest.raises(ValueError):
         oracle.spin_body(pos, vel, 0, 343, nfe, mfe, 1.0, total, (0.0, 0.0, 0.0, w))
+
+
+# ---- Earth / Moon finder (impactBasicFunctions.h:109-217) and final sums ----
+
+def _touch_components(pos, touch=1.5):
+    """Connected components of the reference's float touch test, labelled by
+    their smallest index (independent of the oracle's flood fill)."""
+    from scipy.sparse import csr_matrix
+    from scipy.sparse.csgraph import connected_components
+
+    p = np.asarray(pos, np.float32)[:, :3]
+    dx = p[:, None, 0] - p[None, :, 0]
+    dy = p[:, None, 1] - p[None, :, 1]
+    dz = p[:, None, 2] - p[None, :, 2]
+    mag = np.sqrt((dx * dx + dy * dy) + dz * dz)  # float32 throughout, no contraction
+    adj = csr_matrix(mag < np.float32(touch))
+    _, lab = connected_components(adj, directed=False)
+    first = {}
+    for i, l in enumerate(lab):
+        first.setdefault(l, i)
+    return np.array([first[l] for l in lab], np.int32)
+
+
+@pytest.mark.parametrize("seed,n,box", [(1, 300, 9.0), (2, 400, 14.0), (3, 250, 5.0)])
+def test_earth_moon_groups_are_the_touch_components(oracle, seed, n, box):
+    rng = np.random.default_rng(seed)
+    pos = np.zeros((n, 4), np.float32)
+    pos[:, :3] = rng.uniform(0, box, (n, 3))
+    r = oracle.find_earth_moon(pos)
+    want = _touch_components(pos)
+    assert np.array_equal(r["group"], want)
+    sizes = np.bincount(want, minlength=n)
+    assert r["groups"] == int((sizes > 0).sum())
+    order = sorted(range(n), key=lambda g: (-sizes[g], g))
+    assert r["earth_id"] == order[0] and r["earth_count"] == sizes[order[0]]
+    if sizes[order[1]] > 0:
+        assert r["moon_id"] == order[1] and r["moon_count"] == sizes[order[1]]
+
+
+def _blob(rng, k, centre):
+    """k points on a jittered unit lattice around centre (one touch group)."""
+    side = int(np.ceil(k ** (1 / 3)))
+    g = np.stack(np.meshgrid(*[np.arange(side)] * 3, indexing="ij"), -1).reshape(-1, 3)[:k]
+    return g + rng.uniform(-0.1, 0.1, (k, 3)) + np.asarray(centre)
+
+
+def test_earth_moon_ties_go_to_the_smaller_group_number(oracle):
+    rng = np.random.default_rng(5)
+    a, b, c = _blob(rng, 25, (0, 0, 0)), _blob(rng, 40, (50, 0, 0)), _blob(rng, 40, (0, 50, 0))
+    pos = np.zeros((105, 4), np.float32)
+    pos[:, :3] = np.concatenate([a, b, c])
+    r = oracle.find_earth_moon(pos)
+    assert r["groups"] == 3
+    assert (r["earth_id"], r["earth_count"]) == (25, 40)  # b, first of the two 40s
+    assert (r["moon_id"], r["moon_count"]) == (65, 40)
+    one = oracle.find_earth_moon(pos[:25])
+    assert one["groups"] == 1 and one["earth_count"] == 25 and one["moon_id"] == -1 and one["moon_count"] == 0
+
+
+def test_final_stats_of_a_rigid_rotation(oracle):
+    """A drifting, rigidly rotating blob: centre, mean velocity and L = I w."""
+    rng = np.random.default_rng(9)
+    x = _blob(rng, 343, (3.0, -2.0, 5.0))
+    n, nfe = len(x), 100
+    m = np.where(np.arange(n) < nfe, 2.364458, 1.0)
+    c = (m[:, None] * x).sum(0) / m.sum()
+    w = np.array([0.02, -0.03, 0.05])
+    u = np.array([0.3, 0.1, -0.2])
+    v = np.cross(w, x - c) + u
+    pos, vel = np.zeros((n, 4), np.float32), np.zeros((n, 4), np.float32)
+    pos[:, :3], vel[:, :3] = x, v
+    group = np.zeros(n, np.int32)
+    r = x.astype(np.float32).astype(np.float64) - c
+    I = (m[:, None, None] * ((r * r).sum(1)[:, None, None] * np.eye(3) - r[:, :, None] * r[:, None, :])).sum(0)
+    for fp64, tol in ((True, 1e-6), (False, 2e-3)):
+        s = oracle.final_stats(pos, vel, nfe, group, 0, -1, 2.364458, fp64=fp64)
+        assert s["mass"][0] == pytest.approx(m.sum(), rel=1e-6)
+        assert np.allclose(s["com"][0], c, rtol=0, atol=1e-4)
+        assert np.allclose(s["vel"][0], u, rtol=0, atol=tol * 0.1)
+        assert np.allclose(s["angmom"][0], I @ w, rtol=tol, atol=tol * np.abs(I @ w).max())
+        assert np.allclose(s["angmom"][2], s["angmom"][0], rtol=1e-6)  # the Earth is everything
+        assert s["mass"][3] == 0
+
+
+def test_final_stats_scope_one_is_the_union(oracle):
+    """Masses and momenta add; L of the union = L(Earth) + L(Moon) + the
+    orbital term sum M_s (c_s - c) x (u_s - u) (parallel-axis identity)."""
+    rng = np.random.default_rng(11)
+    a, b = _blob(rng, 125, (0, 0, 0)), _blob(rng, 64, (30, 4, -2))
+    x = np.concatenate([a, b, rng.uniform(60, 70, (5, 3))])
+    n = len(x)
+    pos, vel = np.zeros((n, 4), np.float32), np.zeros((n, 4), np.float32)
+    pos[:, :3] = x
+    vel[:, :3] = rng.normal(0, 1, (n, 3))
+    r = oracle.find_earth_moon(pos)
+    assert r["earth_id"] == 0 and r["moon_id"] == 125
+    s = oracle.final_stats(pos, vel, 40, r["group"], r["earth_id"], r["moon_id"], 2.364458, fp64=True)
+    M, C, U, L = s["mass"], s["com"], s["vel"], s["angmom"]
+    assert M[1] == pytest.approx(M[2] + M[3], rel=1e-14)
+    assert np.allclose(M[1] * U[1], M[2] * U[2] + M[3] * U[3], rtol=1e-12, atol=1e-12)
+    orb = sum(M[k] * np.cross(C[k] - C[1], U[k] - U[1]) for k in (2, 3))
+    assert np.allclose(L[1], L[2] + L[3] + orb, rtol=1e-10, atol=1e-9 * np.abs(L[1]).max())
