@@ -229,6 +229,7 @@ static long long sim_chunk_len(long long n) {
 // about N > 430,000 at R = 8, 300,000 at R = 4) takes the one-sided pass.  The choice depends
 // on the simulation alone, so its sums are the same in any batch.
 constexpr long long kJsideMaxFloatsPerSim = 1LL << 29;
+constexpr long long kMutualMaxChunk = 4096;
 
 // Mutual pass: padded layout, rows of B = 64 R, one diagonal task per row and
 // the row's j-range beyond it cut (at the iron/silicate border) into chunks
@@ -263,7 +264,8 @@ static int plan_mutual(std::vector<SimDesc>& sims, const Tiling& tl, bool batche
         L = (off_total + tt - 1) / tt;
         if (L < tl.min_chunk) L = tl.min_chunk;
     }
-    L = std::max(64LL, (L + 63) / 64 * 64);
+    // at most 4096 j per task: its float i-side sums stay short (see k_mutual)
+    L = std::min(kMutualMaxChunk, std::max(64LL, (L + 63) / 64 * 64));
     std::vector<MutualTask> tasks;
     std::vector<RowDesc> rows;
     int slot = 0;
