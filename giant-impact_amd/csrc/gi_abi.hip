@@ -92,6 +92,7 @@ struct Plan {
     std::vector<GravTask> tasks;
     std::vector<int> block_sim, block_first, sim_first_block;
     Tiling tiling{4, 0, 0, 0, 0};
+    int plain_R = 4;  // i-particles per lane of the one-sided kernel's tasks
     int ntot = 0;
     int partial_len = 0;  // float4 elements
     int buckets = 0;      // contact-grid buckets over all simulations
@@ -213,7 +214,7 @@ static bool same_tiling(const Tiling& a, const Tiling& b) {
 // is too large for it takes the one-sided pass with the default task count.
 static long long sim_chunk_len(long long n) {
     Tiling t = choose_tiling(n, n * n, 1);
-    if (t.variant == 2) t = Tiling{t.R, 0, 0, 32768, 512};
+    if (t.variant == 2) t = Tiling{1, 0, 0, 32768, 512};
     long long L = t.chunk_len;
     if (L <= 0) {
         const long long ib = 64LL * t.R;
@@ -225,10 +226,11 @@ static long long sim_chunk_len(long long n) {
 }
 
 // j-side slabs of the mutual pass grow as 3 N^2 / (128 R) floats per
-// simulation; a simulation whose slab would exceed this many floats (2 GiB:
-// about N > 430,000 at R = 8, 300,000 at R = 4) takes the one-sided pass.  The choice depends
-// on the simulation alone, so its sums are the same in any batch.
-constexpr long long kJsideMaxFloatsPerSim = 1LL << 29;
+// simulation; a simulation whose slab would exceed this many floats (16 GiB:
+// about N > 1.2 M at R = 8, 860,000 at R = 4) takes the one-sided pass.  The
+// choice depends on the simulation alone, so its sums are the same in any
+// batch.
+constexpr long long kJsideMaxFloatsPerSim = 1LL << 32;
 constexpr long long kMutualMaxChunk = 4096;
 
 // Mutual pass: padded layout, rows of B = 64 R, one diagonal task per row and
@@ -257,7 +259,7 @@ static int plan_mutual(std::vector<SimDesc>& sims, const Tiling& tl, bool batche
         jb += 3 * (gn * (nrows - 1) / 2);
     }
     if (gb == 0) return 0;
-    if (jb > (1LL << 33)) return set_error(GI_E_TOO_LARGE, "mutual pass: j-side slabs too large (%lld floats)", jb);
+    if (jb > (1LL << 35)) return set_error(GI_E_TOO_LARGE, "mutual pass: j-side slabs too large (%lld floats)", jb);
     long long L = tl.chunk_len;
     if (L <= 0 && !batched) {
         const long long tt = tl.target_tasks > 0 ? tl.target_tasks : 8192;
@@ -316,7 +318,11 @@ static int build_plan(std::vector<SimDesc> sims, const Tiling& tl, bool batched,
     p.caller_sims = sims;
     p.tiling = tl;
     long long ntot = 0;
-    const int ib = 64 * tl.R;
+    // One-sided tasks use the tiling's R, except beside the mutual pass, where
+    // the one-sided pass is a fallback for simulations too large for it and
+    // takes R = 1 (the one-sided optimum on gfx950).
+    p.plain_R = tl.variant == 2 ? 1 : tl.R;
+    const int ib = 64 * p.plain_R;
     for (auto& s : sims) {
         long long n = s.end - s.start;
         if (n < 0 || s.fe_end < s.start || s.fe_end > s.end || s.start != ntot)
@@ -555,11 +561,11 @@ static int launch_gravity(DevCtx* ctx, Workspace* ws, const float4* pos, bool po
     if (nplain > 0) {
         const dim3 grid(nblk(nplain, 4)), block(256);
         if (pot)
-            launch_gravity_kernel<true, 0>(p.tiling.R, grid, block, st, pos, ws, nplain);
+            launch_gravity_kernel<true, 0>(p.plain_R, grid, block, st, pos, ws, nplain);
         else if (p.tiling.variant == 1)
-            launch_gravity_kernel<false, 1>(p.tiling.R, grid, block, st, pos, ws, nplain);
+            launch_gravity_kernel<false, 1>(p.plain_R, grid, block, st, pos, ws, nplain);
         else
-            launch_gravity_kernel<false, 0>(p.tiling.R, grid, block, st, pos, ws, nplain);
+            launch_gravity_kernel<false, 0>(p.plain_R, grid, block, st, pos, ws, nplain);
     }
     if (nmut > 0) {
         const dim3 grid(nblk(nmut, kMutualWaves)), block(64 * kMutualWaves);

@@ -97,6 +97,8 @@ def main(argv=None) -> int:
     ap.add_argument("--quick", action="store_true", help="fewer candidates (smoke run)")
     ap.add_argument("--out", default=str(TUNING_DIR / "gfx950.txt"))
     ap.add_argument("--max-class", type=int, default=len(CLASSES))
+    ap.add_argument("--classes", default="", help="comma-separated class indices to tune (default: all up to "
+                    "--max-class); rows of the other classes are kept from the current table")
     ap.add_argument("--variants", action="store_true", help="also sweep the rsq*rcp variant")
     ap.add_argument("--batched", action="store_true", help="also tune the batched engine (64-simulation ensemble)")
     args = ap.parse_args(argv)
@@ -111,7 +113,19 @@ def main(argv=None) -> int:
     targets = [4096, 8192, 16384, 32768, 65536, 131072] if not args.quick else [32768]
     mutual_Ls = [256, 512, 1024, 2048] if not args.quick else [512]
     rows = []
-    for n_max, nt, ni in CLASSES[: args.max_class]:
+    only = {int(x) for x in args.classes.split(",") if x.strip()}
+    kept = {}
+    cur = TUNING_DIR / "gfx950.txt"
+    if cur.exists():
+        for line in cur.read_text().splitlines():
+            parts = line.split()
+            if parts and not parts[0].startswith("#") and parts[0] != "B":
+                kept[int(parts[0])] = tuple(int(x) for x in parts)
+    for ci, (n_max, nt, ni) in enumerate(CLASSES[: args.max_class]):
+        if only and ci not in only:
+            if n_max in kept:
+                rows.append(kept[n_max])
+            continue
         run, n = time_gravity(nt, ni, reps=3 if nt + ni > 100000 else 10)
         mc = 128 if n < 4096 else 512
         # (R, variant, chunk_len, target_tasks, min_chunk)
@@ -141,6 +155,10 @@ def main(argv=None) -> int:
         if batched:
             f.write("# batched engine: B chunk_len R variant\n")
             f.write(f"B {batched[0]} {batched[1]} {batched[2]}\n")
+        elif cur.exists():  # keep the current batched row
+            for line in cur.read_text().splitlines():
+                if line.startswith("# batched") or line.startswith("B "):
+                    f.write(line + "\n")
     print(f"wrote {out}")
     return 0
 

@@ -164,3 +164,57 @@ def test_mutual_batched_is_bitwise_the_same_alone_and_in_the_mix(gpu, batched_mu
         assert np.array_equal(p_all[a:b], p1), s
         assert np.array_equal(v_all[a:b], v1), s
     ens.close()
+
+
+def test_mixed_plan_too_large_simulation_takes_the_one_sided_pass(gpu, batched_mutual):
+    """A simulation whose j-side slab would pass the per-simulation cap
+    (N = 1.3 M) takes the one-sided pass inside the same batched launch as two
+    small mutual-pass simulations; each small one stays bitwise the same as
+    alone, and the large one agrees with its single-simulation path."""
+    cfg = [dict(n_target=3000, n_impactor=1000, seed=1, angle_deg=40.0, speed_vesc=1.1, target_spin=0.2),
+           dict(n_target=975000, n_impactor=325000, seed=2, angle_deg=45.0, speed_vesc=1.1, target_spin=0.0),
+           dict(n_target=4000, n_impactor=1200, seed=3, angle_deg=50.0, speed_vesc=1.2, target_spin=0.4)]
+    pos, vel, n, fc, mc, _ = gpu.make_ensemble(cfg)
+    print("n =", list(n), flush=True)
+    P, V = gpu.DeviceArray(len(pos), pos), gpu.DeviceArray(len(pos), vel)
+    ens = gpu.Ensemble(P, V, n, fc, mc)
+    ens.steps(1)
+    gpu.sync()
+    p_all, v_all = P.download(), V.download()
+    off = ens.offsets
+    ens.close()
+    for s in (0, 2):
+        a, b = off[s], off[s + 1]
+        p1, v1 = _ensemble_of_one(gpu, pos[a:b], vel[a:b], fc[s], mc[s], 1)
+        assert np.array_equal(p_all[a:b], p1) and np.array_equal(v_all[a:b], v1), s
+    a, b = off[1], off[2]
+    nb = b - a
+    Pb, Vb, Fb = gpu.DeviceArray(nb, pos[a:b]), gpu.DeviceArray(nb, vel[a:b]), gpu.DeviceArray(nb, f4(nb))
+    gpu.impact_steps(Pb, Vb, Fb, nb, fc[1], mc[1], 1)
+    gpu.sync()
+    assert not gpu.plan_info()["mutual"]  # the single-simulation path falls back too
+    dv = v_all[a:b, :3].astype(np.float64) - Vb.download()[:, :3]
+    kick = v_all[a:b, :3].astype(np.float64) - vel[a:b, :3]
+    assert np.abs(dv).max() <= 1e-5 * np.abs(kick).max()
+
+
+def test_large_simulation_mutual_pass_against_fp64(gpu, oracle):
+    """N = 1,048,576 with the tuning table's tiling (the mutual pass): rows
+    across every material against the FP64 oracle and the reference's float
+    routine, and Newton's third law.  At this size float positions (|x| up to
+    ~300) alone limit near-pair separations to ~1e-5 relative; the reference
+    float routine's own error here is 6e-4 to 1e-2."""
+    pos, vel, fc, _, _ = gpu.make_impact(786432, 262144, seed=6, angle_deg=45.0, speed_vesc=1.1)
+    n = len(pos)
+    print(f"n={n}", flush=True)
+    F = _force(gpu, pos, vel, fc)
+    assert gpu.plan_info()["mutual"]
+    for a in (0, fc.NFe - 100, fc.NFe + 4000, n - 200):
+        F64, cond, near = oracle.force_collision_f64(pos, vel, fc, i_begin=a, i_end=a + 128)
+        Fr = oracle.force_collision_f32(pos, vel, fc, a, a + 128)
+        sl = slice(a, a + 128)
+        met = force_metrics(F[sl], F64[sl], cond[sl], near[sl], Fr[sl])
+        print(a, met)
+        assert_force_parity(met)
+    tot = F[:, :3].astype(np.float64).sum(0)
+    assert np.abs(tot).max() <= 1e-5 * np.abs(F[:, :3]).astype(np.float64).sum()
