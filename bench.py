@@ -83,12 +83,14 @@ def cpu_baseline(pos, vel, fc, n, target_s: float):
     }
 
 
-def ensemble_leg(gi, n_sims: int, steps: int) -> dict:
+def ensemble_leg(gi, n_sims: int, steps: int, seq_steps: int = 100, check_every: int = 100) -> dict:
     """64 independent impacts of uneven N (U[4096, 32768], impactor fraction
     [0.1, 0.3], angle [30, 60] deg, speed [1.0, 1.4] v_esc, target spin [0, 0.6]
-    breakup), batched vs each simulation stepped alone with the plain path
-    (gi_impact_steps, also graph-captured; its per-simulation set-up is
-    excluded from the sequential time)."""
+    breakup), batched (steps steps, FP64 E / L of every simulation every
+    check_every steps outside the timed region) vs each simulation stepped
+    alone with the single-simulation path (gi_impact_steps, also
+    graph-captured; seq_steps steps each, its per-simulation set-up excluded
+    from the sequential time); rates are compared per sim-step."""
     cfg = gi.ensemble_configs(n_sims)
     pos, vel, n, fc, mc, _ = gi.make_ensemble(cfg)
     ntot = int(n.sum())
@@ -98,11 +100,23 @@ def ensemble_leg(gi, n_sims: int, steps: int) -> dict:
     c0 = ens.conserved()
     ens.steps(1)
     gi.sync()
-    t0 = time.perf_counter()
-    ens.steps(steps)
-    gi.sync()
-    t_batched = time.perf_counter() - t0
-    c1 = ens.conserved()
+    # batched: `steps` steps in segments of check_every, FP64 E / L of every
+    # simulation between segments (outside the timed region)
+    t_batched = 0.0
+    done = 0
+    dE_series, dL_series = [], []
+    scaleL = np.maximum(np.linalg.norm(c0[:, 6:9], axis=1), 1e-30)
+    while done < steps:
+        k = min(check_every, steps - done)
+        t0 = time.perf_counter()
+        ens.steps(k)
+        gi.sync()
+        t_batched += time.perf_counter() - t0
+        done += k
+        c = ens.conserved()
+        dE_series.append(float((np.abs(c[:, 2] - c0[:, 2]) / np.abs(c0[:, 2])).max()))
+        dL_series.append(float((np.linalg.norm(c[:, 6:9] - c0[:, 6:9], axis=1) / scaleL).max()))
+    c1 = c
     ens.close()
     # sequential: every simulation alone, one after another
     P.upload(pos)
@@ -115,12 +129,13 @@ def ensemble_leg(gi, n_sims: int, steps: int) -> dict:
         gi.impact_steps(ptr(P), ptr(V), ptr(F), int(n[s]), fc[s], mc[s], 1)  # plan + capture
         gi.sync()
         t0 = time.perf_counter()
-        gi.impact_steps(ptr(P), ptr(V), ptr(F), int(n[s]), fc[s], mc[s], steps)
+        gi.impact_steps(ptr(P), ptr(V), ptr(F), int(n[s]), fc[s], mc[s], seq_steps)
         gi.sync()
         t_seq += time.perf_counter() - t0
     dE = np.abs(c1[:, 2] - c0[:, 2]) / np.abs(c0[:, 2])
-    scaleL = np.maximum(np.linalg.norm(c0[:, 6:9], axis=1), 1e-30)
     dL = np.linalg.norm(c1[:, 6:9] - c0[:, 6:9], axis=1) / scaleL
+    batched_rate = n_sims * steps / t_batched
+    seq_rate = n_sims * seq_steps / t_seq
     return {
         "sims": n_sims,
         "particles": ntot,
@@ -128,12 +143,16 @@ def ensemble_leg(gi, n_sims: int, steps: int) -> dict:
         "n_max": int(n.max()),
         "pairs_per_step": pairs,
         "steps": steps,
-        "batched_sim_steps_per_s": n_sims * steps / t_batched,
-        "sequential_sim_steps_per_s": n_sims * steps / t_seq,
-        "speedup_vs_sequential": t_seq / t_batched,
+        "sequential_steps": seq_steps,
+        "batched_sim_steps_per_s": batched_rate,
+        "sequential_sim_steps_per_s": seq_rate,
+        "speedup_vs_sequential": batched_rate / seq_rate,
         "batched_pair_per_s": pairs * steps / t_batched,
+        "fp64_check_every": check_every,
         "fp64_energy_rel_change_max": float(dE.max()),
         "fp64_angmom_rel_change_max": float(dL.max()),
+        "fp64_energy_rel_change_series": dE_series,
+        "fp64_angmom_rel_change_series": dL_series,
     }
 
 
@@ -236,7 +255,8 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--ensemble-sims", type=int, default=64, help="0 skips the ensemble leg")
-    ap.add_argument("--ensemble-steps", type=int, default=100)
+    ap.add_argument("--ensemble-steps", type=int, default=1000,
+                    help="batched steps of the ensemble leg (FP64 E / L every 100)")
     ap.add_argument("--only-ensemble", action="store_true", help="run the ensemble leg alone (profiling)")
     ap.add_argument("--serve-requests", type=int, default=24,
                     help="requests of the serving leg (0 = skip)")

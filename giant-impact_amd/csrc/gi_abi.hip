@@ -26,6 +26,7 @@ static_assert(sizeof(gi_move_separate_constants) == 24, "moveSeperateKernalConst
 namespace gi {
 
 static thread_local std::string g_last_error;
+static thread_local int g_last_code = 0;
 
 static int set_error(int code, const char* fmt, ...) {
     char buf[512];
@@ -34,6 +35,7 @@ static int set_error(int code, const char* fmt, ...) {
     vsnprintf(buf, sizeof buf, fmt, ap);
     va_end(ap);
     g_last_error = buf;
+    g_last_code = code;
     return code;
 }
 
@@ -777,6 +779,13 @@ int gi_abi_version(void) { return GI_ABI_VERSION; }
 
 const char* gi_last_error_string(void) { return g_last_error.c_str(); }
 
+int gi_last_error(void) { return g_last_code; }
+
+void gi_clear_error(void) {
+    g_last_code = 0;
+    g_last_error.clear();
+}
+
 int gi_device_count(void) {
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess) return 0;
@@ -842,6 +851,8 @@ int gi_device_sync(void) {
     GI_HIP(hipDeviceSynchronize());
     return 0;
 }
+
+int gi_sync(void) { return gi_device_sync(); }
 
 int gi_tiling_for(long long n, int* r, int* variant, long long* chunk_len, long long* target_tasks,
                   long long* min_chunk) {

@@ -212,6 +212,9 @@ def load() -> C.CDLL:
     sig = {
         "gi_abi_version": (i, []),
         "gi_last_error_string": (C.c_char_p, []),
+        "gi_last_error": (i, []),
+        "gi_clear_error": (None, []),
+        "gi_sync": (i, []),
         "gi_device_count": (i, []),
         "gi_set_device": (i, [i]),
         "gi_check_block_multiple": (i, [i]),

@@ -128,6 +128,12 @@ typedef struct gi_move_separate_constants {
 
 int gi_abi_version(void);
 const char *gi_last_error_string(void);
+/* Code of the failure gi_last_error_string() describes (0 when none since the
+ * last gi_clear_error on this thread); the reference's cudaErrorCheck
+ * (Code/CommonCode/commonFunctions.h:12-22) would check it instead of
+ * cudaGetLastError. */
+int gi_last_error(void);
+void gi_clear_error(void);
 /* Number of visible HIP devices (0 when none); never fails. */
 int gi_device_count(void);
 /* Select the device used by subsequent calls from this thread. */
@@ -147,6 +153,8 @@ int gi_stream_create(gi_stream_t *stream);
 int gi_stream_destroy(gi_stream_t stream);
 int gi_stream_sync(gi_stream_t stream);
 int gi_device_sync(void);
+/* Same as gi_device_sync: the cudaDeviceSynchronize the reference calls. */
+int gi_sync(void);
 /* Free and total device memory in bytes (hipMemGetInfo). */
 int gi_mem_info(size_t *free_bytes, size_t *total_bytes);
 /* The plan of the last drop-in force call on the current device: mutual = 1
