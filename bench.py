@@ -195,13 +195,14 @@ def serving_leg(gi, n_requests: int, mean_gap_s: float, seed: int = 20261206) ->
     }
 
 
-def drift_leg(gi, steps_small: int, steps_large: int, check_every: int) -> dict:
+def drift_leg(gi, steps_small: int, steps_large: int, check_every: int, steps_headline: int = 0) -> dict:
     """Conserved quantities over long runs.  N = 2,048 (1,536 + 512, 45 deg,
     1.2 v_esc): the GPU and the oracle's float restatement of the reference
     kernels both run steps_small steps from the same state and their FP64
     E / P / L changes are compared.  N = 65,536 (52,429 + 13,107, target spin
     0.5 breakup, 45 deg, 1.1 v_esc): steps_large steps on the GPU with FP64
-    E / P / L every check_every steps.  Energy is not conserved by this law
+    E / P / L every check_every steps.  The benchmark impact (N = 262,144):
+    steps_headline steps with FP64 E / P / L every quarter.  Energy is not conserved by this law
     (receding contacts use the reduced stiffness KR), angular and linear
     momentum are."""
     from oracle import oracle as O
@@ -242,6 +243,22 @@ def drift_leg(gi, steps_small: int, steps_large: int, check_every: int) -> dict:
                      "final": series[-1],
                      "angmom_rel_change_max": max(x["angmom_rel_change"] for x in series),
                      "energy_rel_change_series": [x["energy_rel_change"] for x in series]}
+    if steps_headline > 0:
+        # the benchmark impact itself over its configured run length
+        pos, vel, fc, mc, _ = gi.make_impact(196608, 65536, seed=20261202 + 2, angle_deg=45.0, speed_vesc=1.1)
+        n = len(pos)
+        P, V, F = gi.DeviceArray(n, pos), gi.DeviceArray(n, vel), gi.DeviceArray(n, np.zeros((n, 4), np.float32))
+        c0 = gi.conserved_quantities(P, V, n, fc)
+        series = []
+        t0 = time.perf_counter()
+        every = max(1, steps_headline // 4)
+        for k in range(0, steps_headline, every):
+            gi.impact_steps(P, V, F, n, fc, mc, min(every, steps_headline - k))
+            series.append(rel(c0, gi.conserved_quantities(P, V, n, fc)))
+        out["n262144"] = {"steps": steps_headline, "check_every": every, "wall_s": time.perf_counter() - t0,
+                          "final": series[-1],
+                          "angmom_rel_change_max": max(x["angmom_rel_change"] for x in series),
+                          "energy_rel_change_series": [x["energy_rel_change"] for x in series]}
     return out
 
 
@@ -263,6 +280,8 @@ def main() -> int:
     ap.add_argument("--serve-gap", type=float, default=1.5, help="mean Poisson inter-arrival gap (s)")
     ap.add_argument("--drift-steps", type=int, default=10000,
                     help="steps of the N=65,536 conserved-quantity run (0 = skip the drift leg)")
+    ap.add_argument("--drift-headline-steps", type=int, default=2000,
+                    help="steps of the N=262,144 conserved-quantity run (0 = skip)")
     ap.add_argument("--profile-probe", action="store_true",
                     help="two steps + two 64 MiB copies, for hardware-counter passes")
     args = ap.parse_args()
@@ -412,8 +431,9 @@ def main() -> int:
         result["serving"] = serving_leg(gi, args.serve_requests, args.serve_gap)
         log(f"serving: {result['serving']}")
     if args.drift_steps > 0:
-        log(f"drift leg: N=2048 x 1000 steps against the oracle, N=65536 x {args.drift_steps} steps")
-        result["drift"] = drift_leg(gi, 1000, args.drift_steps, 1000)
+        log(f"drift leg: N=2048 x 1000 steps against the oracle, N=65536 x {args.drift_steps} steps, "
+            f"N=262144 x {args.drift_headline_steps} steps")
+        result["drift"] = drift_leg(gi, 1000, args.drift_steps, 1000, args.drift_headline_steps)
         log(f"drift: {result['drift']}")
     if not args.no_cpu:
         log("timing the CPU baseline")
