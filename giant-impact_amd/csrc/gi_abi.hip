@@ -206,14 +206,11 @@ static bool same_tiling(const Tiling& a, const Tiling& b) {
            a.target_tasks == b.target_tasks && a.min_chunk == b.min_chunk;
 }
 
-// Fill nchunk / partial_base of the simulations and build the task table.
-// j-chunk length of one simulation.  It depends on that simulation's own size
-// only (through its row of the tuning table), never on the rest of a batch:
-// chunk boundaries fix the order of every float sum, so a simulation gives
-// bitwise the same forces alone and inside any ensemble.
-// One-sided chunk length of a simulation of n particles planned alone (its
-// size's tuning row).  A simulation whose row asks for the mutual pass but
-// is too large for it takes the one-sided pass with the default task count.
+// One-sided j-chunk length of a simulation of n particles planned alone.  It
+// depends on that simulation's own size only (through its row of the tuning
+// table), never on the rest of the launch: chunk boundaries fix the order of
+// every float sum.  A simulation whose row asks for the mutual pass but is
+// too large for it takes the one-sided pass with the default task count.
 static long long sim_chunk_len(long long n) {
     Tiling t = choose_tiling(n, n * n, 1);
     if (t.variant == 2) t = Tiling{1, 0, 0, 32768, 512};
@@ -315,6 +312,11 @@ static int plan_mutual(std::vector<SimDesc>& sims, const Tiling& tl, bool batche
     return 0;
 }
 
+// The plan of one force evaluation over `sims`: the mutual pass's tasks and
+// rows for the simulations that take it, then the one-sided pass's chunks
+// and tasks for the rest, the contact-grid bucket ranges and the moment
+// blocks.  batched: every simulation uses the fixed tiling of the batched
+// engine, so its sums are the same alone and in any batch.
 static int build_plan(std::vector<SimDesc> sims, const Tiling& tl, bool batched, Plan* out) {
     Plan p;
     p.caller_sims = sims;

@@ -5,17 +5,18 @@
 // (getForcesCollisionSingleGPU, Code/ImpactCode/impactAddTime.cu:245-377).
 // Here the same force is produced by three passes:
 //
-//   1. k_gravity  -- all pairs, branch free: sum_j (G mi mj) dp / r^3 with a
-//      hardware rsq, i-particles in registers (R per lane), j-particles read
-//      through wave-uniform scalar loads, float sums per j-tile folded into a
-//      Kahan-compensated accumulator, one j-chunk per wavefront, partial sums
-//      to a slot buffer.
+//   1. all pairs, branch free: sum_j (G mi mj) dp / r^3 with a hardware rsq,
+//      in one of two forms chosen per simulation by the tuning table:
+//      k_gravity (one-sided: i-particles in registers, R per lane,
+//      j-particles through wave-uniform scalar loads, Kahan-folded float
+//      tiles, one j-chunk per wavefront, partial sums to a slot buffer), or
+//      the mutual pass (k_mutual_pad, k_mutual, k_mutual_sum: each pair
+//      evaluated once for both particles, per-particle sums in FP64).
 //   2. the contact grid -- a spatial hash of cells of size 1 (the element
 //      diameter) built by counting sort (k_grid_*).
-//   3. k_finish   -- per particle: sum of the j-chunk partials in fixed order
-//      plus, for every pair with r < 1 found through the grid, the
-//      reference's exact piecewise contact force minus what pass 1 added for
-//      that pair.
+//   3. k_finish   -- per particle: its all-pairs sum plus, for every pair
+//      with r < 1 found through the grid, the reference's exact piecewise
+//      contact force minus what pass 1 added for that pair.
 //
 // Only pairs with r < 1 can differ from gravity (impactAddTime.cu:300,320,349),
 // so the sum equals the reference's force up to float rounding, and the
