@@ -42,6 +42,14 @@ struct Job {
 // of at most a few chunks, contact grid, bucket-ordered copies, potentials).
 constexpr long long kBytesPerParticle = 2 * 32 + 16 + 4 * 16 + 16 + 32 + 8 + 16;
 
+// The mutual all-pairs pass also keeps, per simulation, a j-side slab and
+// i-side slots that grow as N^2: about 0.012 (N + 1024)^2 bytes each at rows
+// of 512 and chunks of 512 (padded N).  Admission counts 0.03 (N + 1024)^2.
+double quadratic_bytes(int n) {
+    const double g = (double)n + 1024.0;
+    return 0.03 * g * g;
+}
+
 struct DevArrays {
     void *pos = nullptr, *vel = nullptr;
 };
@@ -73,15 +81,18 @@ extern "C" int gi_serve_run(const gi_serve_request *reqs, int n_requests, const 
         if (c.n_target <= 0 || c.n_impactor < 0 || reqs[k].steps < 0) return GI_E_INVALID_ARG;
         biggest = std::max<long long>(biggest, (long long)c.n_target + c.n_impactor);
     }
-    long long capacity = o.max_particles;
-    if (capacity <= 0) {
-        size_t free_b = 0, total_b = 0;
+    size_t free_b = 0, total_b = 0;
+    {
         int rc = gi_mem_info(&free_b, &total_b);
         if (rc) return rc;
-        capacity = (long long)(o.hbm_fraction * (double)free_b) / kBytesPerParticle;
     }
+    const double budget = o.hbm_fraction * (double)free_b;
+    long long capacity = o.max_particles;
+    if (capacity <= 0) capacity = (long long)(0.5 * budget) / kBytesPerParticle;
     capacity = std::min<long long>(capacity, 1LL << 21);
-    if (capacity < biggest) return GI_E_TOO_LARGE;
+    // what is left of the budget for the per-simulation N^2 scratch
+    const double quad_budget = budget - (double)capacity * kBytesPerParticle;
+    if (capacity < biggest || quadratic_bytes((int)biggest) > quad_budget) return GI_E_TOO_LARGE;
 
     DevArrays buf[2];
     int rc = 0;
@@ -105,6 +116,7 @@ extern "C" int gi_serve_run(const gi_serve_request *reqs, int n_requests, const 
     size_t next = 0;
     int completed = 0, rounds = 0, peak_sims = 0;
     long long resident = 0, peak_resident = 0;
+    double resident_quad = 0.0;
     double total_steps = 0.0, total_pairs = 0.0;
     const Clock::time_point t0 = Clock::now();
 
@@ -180,7 +192,9 @@ extern "C" int gi_serve_run(const gi_serve_request *reqs, int n_requests, const 
         while (next < order.size() && reqs[order[next]].arrival_s <= now) {
             const gi_serve_request &r = reqs[order[next]];
             const int n = r.impact.n_target + r.impact.n_impactor;
-            if ((int)active.size() >= o.max_sims || resident + n > capacity) break;  // wait for room, FIFO
+            if ((int)active.size() >= o.max_sims || resident + n > capacity ||
+                resident_quad + quadratic_bytes(n) > quad_budget)
+                break;  // wait for room, FIFO
             Job j;
             j.idx = order[next];
             j.n = n;
@@ -193,6 +207,7 @@ extern "C" int gi_serve_run(const gi_serve_request *reqs, int n_requests, const 
             results[j.idx].arrival_s = r.arrival_s;
             results[j.idx].admit_s = now;
             resident += n;
+            resident_quad += quadratic_bytes(n);
             active.push_back(std::move(j));
             ++next;
             changed = true;
@@ -255,6 +270,7 @@ extern "C" int gi_serve_run(const gi_serve_request *reqs, int n_requests, const 
             if (r.final_vel && (rc = gi_copy_d2h(r.final_vel, (char *)buf[cur].vel + (size_t)j.offset * 16, bytes)))
                 return fail(rc);
             resident -= j.n;
+            resident_quad -= quadratic_bytes(j.n);
             ++completed;
         }
         active = std::move(keep);
