@@ -1295,16 +1295,16 @@ int gi_ensemble_create(gi_ensemble** out, int n_sims, const int* n, const gi_for
             delete e;
             return set_error(GI_E_INVALID_ARG, "gi_ensemble_create: simulation %d has invalid n/NFe", s);
         }
+        if ((long long)off + n[s] > (1LL << 21)) {
+            delete e;
+            return set_error(GI_E_TOO_LARGE, "gi_ensemble_create: more than 2^21 particles");
+        }
         SimDesc d = make_sim(off, off + fc[s].NFe, off + n[s], law_of(&fc[s]));
         d.dt = mc[s].Dt;
         d.dt_over_mass_fe = mc[s].DtOverMassFe;
         d.dt_over_mass_si = mc[s].DtOverMassSi;
         e->sims.push_back(d);
         off += n[s];
-        if (off > (1 << 21)) {
-            delete e;
-            return set_error(GI_E_TOO_LARGE, "gi_ensemble_create: more than 2^21 particles");
-        }
     }
     e->pos = (float4*)pos;
     e->vel = (float4*)vel;
