@@ -293,6 +293,9 @@ def main() -> int:
                     help="steps of the N=262,144 conserved-quantity run (0 = skip)")
     ap.add_argument("--profile-probe", action="store_true",
                     help="two steps + two 64 MiB copies, for hardware-counter passes")
+    ap.add_argument("--probe-split", type=int, default=1,
+                    help="profile probe: launch the all-pairs task list as this many dispatches "
+                         "(counters cover a bounded number of wavefronts per dispatch)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -316,16 +319,25 @@ def main() -> int:
         return 0
 
     if args.profile_probe:
-        # Short fixed workload for hardware-counter passes: two force + move
-        # steps of the benchmark impact, then two 64 MiB device copies whose
-        # known byte counts calibrate FETCH_SIZE / WRITE_SIZE.
+        # Short fixed workload for hardware-counter passes: one force call
+        # (plan), two force + move steps of the benchmark impact with the
+        # all-pairs task list cut into --probe-split dispatches, then two
+        # 64 MiB device copies whose known byte counts calibrate FETCH_SIZE /
+        # WRITE_SIZE.
         pos, vel, fc, mc, _ = gi.make_impact(args.n_target, args.n_impactor, seed=20261202 + 2, angle_deg=45.0,
                                              speed_vesc=1.1)
         n = len(pos)
         P, V, F = gi.DeviceArray(n, pos), gi.DeviceArray(n, vel), gi.DeviceArray(n, np.zeros((n, 4), np.float32))
+        gi.force_collision(P, V, F, n, fc)  # builds the plan (one dispatch)
+        gi.sync()
+        ntasks = gi.plan_info()["pair_tasks"]
+        split = max(1, args.probe_split)
+        gi.call("gi_set_dispatch_limit", -(-ntasks // split) if split > 1 else 0)
+        log(f"profile probe: {ntasks} all-pair tasks in {split} dispatch(es) per force evaluation")
         for _ in range(2):
             gi.force_collision(P, V, F, n, fc)
             gi.move_collision(P, V, F, n, mc)
+        gi.call("gi_set_dispatch_limit", 0)
         nb = 64 << 20
         a, b = C.c_void_p(), C.c_void_p()
         gi.call("gi_malloc", C.byref(a), nb)

@@ -7,6 +7,8 @@
 #include "gi_kernels.hip"
 #include "gi_earth_moon.hip"
 
+#include <algorithm>
+#include <atomic>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -506,36 +508,46 @@ static TimingPair take_pair(DevCtx* ctx, int kind) {
     return t;
 }
 
+// Wavefront tasks per all-pairs dispatch (0: all in one); see
+// gi_set_dispatch_limit.
+static std::atomic<long long> g_dispatch_limit{0};
+
+// Tasks [t0, t0 + ntasks) of the plan's one-sided list.
 template <bool POT, int V>
-static void launch_gravity_kernel(int R, dim3 grid, dim3 block, hipStream_t st, const float4* pos,
-                                  const Workspace* ws, int ntasks) {
+static void launch_gravity_kernel(int R, hipStream_t st, const float4* pos, const Workspace* ws, int t0,
+                                  int ntasks) {
+    const dim3 grid(nblk(ntasks, 4)), block(256);
+    const GravTask* tasks = ws->tasks.p + t0;
     if (R == 8)
-        hipLaunchKernelGGL((k_gravity<8, 128, POT, V>), grid, block, 0, st, pos, ws->tasks.p, ntasks, ws->sims.p,
+        hipLaunchKernelGGL((k_gravity<8, 128, POT, V>), grid, block, 0, st, pos, tasks, ntasks, ws->sims.p,
                            ws->partial.p);
     else if (R == 2)
-        hipLaunchKernelGGL((k_gravity<2, 128, POT, V>), grid, block, 0, st, pos, ws->tasks.p, ntasks, ws->sims.p,
+        hipLaunchKernelGGL((k_gravity<2, 128, POT, V>), grid, block, 0, st, pos, tasks, ntasks, ws->sims.p,
                            ws->partial.p);
     else if (R == 1)
-        hipLaunchKernelGGL((k_gravity<1, 128, POT, V>), grid, block, 0, st, pos, ws->tasks.p, ntasks, ws->sims.p,
+        hipLaunchKernelGGL((k_gravity<1, 128, POT, V>), grid, block, 0, st, pos, tasks, ntasks, ws->sims.p,
                            ws->partial.p);
     else
-        hipLaunchKernelGGL((k_gravity<4, 128, POT, V>), grid, block, 0, st, pos, ws->tasks.p, ntasks, ws->sims.p,
+        hipLaunchKernelGGL((k_gravity<4, 128, POT, V>), grid, block, 0, st, pos, tasks, ntasks, ws->sims.p,
                            ws->partial.p);
 }
 
+// Tasks [t0, t0 + ntasks) of the plan's mutual list.
 template <bool POT>
-static void launch_mutual_kernel(int R, dim3 grid, dim3 block, hipStream_t st, const Workspace* ws, int ntasks) {
+static void launch_mutual_kernel(int R, hipStream_t st, const Workspace* ws, int t0, int ntasks) {
+    const dim3 grid(nblk(ntasks, kMutualWaves)), block(64 * kMutualWaves);
+    const MutualTask* tasks = ws->mtasks.p + t0;
     if (R == 8)
-        hipLaunchKernelGGL((k_mutual<8, POT>), grid, block, 0, st, ws->gpos.p, ws->mtasks.p, ntasks, ws->sims.p,
+        hipLaunchKernelGGL((k_mutual<8, POT>), grid, block, 0, st, ws->gpos.p, tasks, ntasks, ws->sims.p,
                            ws->iside.p, ws->jside.p);
     else if (R == 4)
-        hipLaunchKernelGGL((k_mutual<4, POT>), grid, block, 0, st, ws->gpos.p, ws->mtasks.p, ntasks, ws->sims.p,
+        hipLaunchKernelGGL((k_mutual<4, POT>), grid, block, 0, st, ws->gpos.p, tasks, ntasks, ws->sims.p,
                            ws->iside.p, ws->jside.p);
     else if (R == 2)
-        hipLaunchKernelGGL((k_mutual<2, POT>), grid, block, 0, st, ws->gpos.p, ws->mtasks.p, ntasks, ws->sims.p,
+        hipLaunchKernelGGL((k_mutual<2, POT>), grid, block, 0, st, ws->gpos.p, tasks, ntasks, ws->sims.p,
                            ws->iside.p, ws->jside.p);
     else
-        hipLaunchKernelGGL((k_mutual<1, POT>), grid, block, 0, st, ws->gpos.p, ws->mtasks.p, ntasks, ws->sims.p,
+        hipLaunchKernelGGL((k_mutual<1, POT>), grid, block, 0, st, ws->gpos.p, tasks, ntasks, ws->sims.p,
                            ws->iside.p, ws->jside.p);
 }
 
@@ -562,21 +574,26 @@ static int launch_gravity(DevCtx* ctx, Workspace* ws, const float4* pos, bool po
         tp = take_pair(ctx, 0);
         GI_HIP(hipEventRecord(tp.a, st));
     }
-    if (nplain > 0) {
-        const dim3 grid(nblk(nplain, 4)), block(256);
+    // Every task writes only its own outputs, so cutting a task list into
+    // several dispatches changes nothing in the results.
+    const long long lim = g_dispatch_limit.load();
+    const int step_plain = lim > 0 && lim < nplain ? (int)lim : std::max(nplain, 1);
+    for (int t0 = 0; t0 < nplain; t0 += step_plain) {
+        const int cnt = std::min(step_plain, nplain - t0);
         if (pot)
-            launch_gravity_kernel<true, 0>(p.plain_R, grid, block, st, pos, ws, nplain);
+            launch_gravity_kernel<true, 0>(p.plain_R, st, pos, ws, t0, cnt);
         else if (p.tiling.variant == 1)
-            launch_gravity_kernel<false, 1>(p.plain_R, grid, block, st, pos, ws, nplain);
+            launch_gravity_kernel<false, 1>(p.plain_R, st, pos, ws, t0, cnt);
         else
-            launch_gravity_kernel<false, 0>(p.plain_R, grid, block, st, pos, ws, nplain);
+            launch_gravity_kernel<false, 0>(p.plain_R, st, pos, ws, t0, cnt);
     }
-    if (nmut > 0) {
-        const dim3 grid(nblk(nmut, kMutualWaves)), block(64 * kMutualWaves);
+    const int step_mut = lim > 0 && lim < nmut ? (int)lim : std::max(nmut, 1);
+    for (int t0 = 0; t0 < nmut; t0 += step_mut) {
+        const int cnt = std::min(step_mut, nmut - t0);
         if (pot)
-            launch_mutual_kernel<true>(p.tiling.R, grid, block, st, ws, nmut);
+            launch_mutual_kernel<true>(p.tiling.R, st, ws, t0, cnt);
         else
-            launch_mutual_kernel<false>(p.tiling.R, grid, block, st, ws, nmut);
+            launch_mutual_kernel<false>(p.tiling.R, st, ws, t0, cnt);
     }
     GI_HIP(hipGetLastError());
     if (timed) {
@@ -896,6 +913,12 @@ int gi_set_batched_tiling(int r, long long chunk_len, int variant) {
         return set_error(GI_E_INVALID_ARG,
                          "gi_set_batched_tiling: r in {1,2,4,8}, chunk_len >= 64, variant in {0,1,2}");
     batched_override() = BatchedTiling{(chunk_len + 63) / 64 * 64, r, variant};
+    return 0;
+}
+
+int gi_set_dispatch_limit(long long max_tasks) {
+    if (max_tasks < 0) return set_error(GI_E_INVALID_ARG, "gi_set_dispatch_limit: max_tasks < 0");
+    g_dispatch_limit.store(max_tasks);
     return 0;
 }
 

@@ -315,6 +315,16 @@ int gi_tiling_for(long long n, int *r, int *variant, long long *chunk_len, long 
  * active set). */
 int gi_set_batched_tiling(int r, long long chunk_len, int variant);
 
+/* Wavefront tasks per all-pairs kernel dispatch: a longer task list is
+ * launched as consecutive dispatches of at most max_tasks tasks on the same
+ * stream (0, the default, = one dispatch).  Results are bitwise the same
+ * either way: every task writes only its own partial sums.  For hardware
+ * counter collection, which covers a bounded number of wavefronts per
+ * dispatch, and for callers that share the GPU with latency-bound work.
+ * Process-wide; applies to launches made (or graphs captured) after the call.
+ * No reference counterpart (the reference launches one N-thread grid). */
+int gi_set_dispatch_limit(long long max_tasks);
+
 /* ---- kernel timing (HIP events around the all-pairs force kernel) ------- */
 
 /* Enable/disable per-launch timing (HIP events on the launching stream) of the

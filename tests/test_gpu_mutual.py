@@ -81,6 +81,37 @@ def test_mutual_agrees_with_the_one_sided_pass(gpu, tiling):
     assert np.abs(tot).max() <= 1e-5 * np.abs(Fm).sum()
 
 
+@pytest.mark.parametrize("variant,R,L", [(2, 8, 128), (2, 4, 256), (0, 4, 512)])
+def test_dispatch_limit_is_bitwise_the_same(gpu, tiling, variant, R, L):
+    """gi_set_dispatch_limit cuts the task list into several dispatches; every
+    task writes only its own partial sums, so force and potential energy are
+    bitwise unchanged (limits: one task, a prime, more than there are)."""
+    pos, vel, fc, _, _ = gpu.make_impact(3000, 1097, seed=3, angle_deg=60.0, speed_vesc=1.4)
+    n = len(pos)
+    print(f"variant={variant} R={R} L={L}", flush=True)
+    tiling(R, variant, L)
+    P, V, F = gpu.DeviceArray(n, pos), gpu.DeviceArray(n, vel), gpu.DeviceArray(n, f4(n))
+
+    def run():
+        gpu.force_collision(P, V, F, n, fc)
+        gpu.sync()
+        ntasks = gpu.plan_info()["pair_tasks"]
+        e = gpu.conserved_quantities(P, V, n, fc)
+        gpu.sync()
+        return F.download(), e, ntasks
+
+    F0, e0, ntasks = run()
+    assert ntasks > 7
+    try:
+        for lim in (1, 7, ntasks + 5):
+            gpu.call("gi_set_dispatch_limit", lim)
+            F1, e1, _ = run()
+            assert np.array_equal(F0, F1), lim
+            assert np.array_equal(e0, e1), lim
+    finally:
+        gpu.call("gi_set_dispatch_limit", 0)
+
+
 @pytest.mark.parametrize("n,nfe", [(1, 0), (2, 1), (3, 3), (65, 0), (257, 100), (640, 640), (1100, 512)])
 def test_mutual_small_and_ragged_sizes(gpu, oracle, tiling, n, nfe):
     """Rows padded with ghosts, one-material simulations, a material border

@@ -139,6 +139,7 @@ def test_invalid_arguments_are_rejected_before_touching_a_device(gi):
     assert lib.gi_check_block_multiple(1000) == -2 and lib.gi_check_block_multiple(1024) == 0
     assert lib.gi_set_tiling(3, 0, 0, 0, 0) == -1 and lib.gi_set_tiling(4, 3, 0, 0, 0) == -1
     assert lib.gi_set_tiling(0, 0, 0, 0, 0) == 0
+    assert lib.gi_set_dispatch_limit(-1) == -1 and lib.gi_set_dispatch_limit(0) == 0
     assert lib.gi_impact_steps(vp(1), vp(1), vp(1), 10, C.byref(fc), C.byref(mc), 5, 1, None, None, None) == -1
     assert lib.gi_last_error() == -1
     lib.gi_clear_error()
