@@ -126,7 +126,8 @@ def ensemble_leg(gi, n_sims: int, steps: int, seq_steps: int = 100, check_every:
     t_seq = 0.0
     for s in range(n_sims):
         ptr = lambda A: A.value + 16 * int(off[s])
-        gi.impact_steps(ptr(P), ptr(V), ptr(F), int(n[s]), fc[s], mc[s], 1)  # plan + capture
+        # plan + capture of the one-step and the multi-step graphs, untimed
+        gi.impact_steps(ptr(P), ptr(V), ptr(F), int(n[s]), fc[s], mc[s], 8)
         gi.sync()
         t0 = time.perf_counter()
         gi.impact_steps(ptr(P), ptr(V), ptr(F), int(n[s]), fc[s], mc[s], seq_steps)
@@ -195,6 +196,28 @@ def serving_leg(gi, n_requests: int, mean_gap_s: float, seed: int = 20261206) ->
     }
 
 
+def spot_parity(gi, pos, vel, fc, P, V, F) -> dict:
+    """One force evaluation on the GPU against the FP64 oracle on 5 x 64 rows
+    spread over the materials and bodies (iron start and middle, the Fe/Si
+    border, the middle, the end): the largest error relative to sum_j |f_ij|
+    and to |F|, rows with a pair at the contact branch points excluded."""
+    from oracle import oracle as O
+
+    n = len(pos)
+    gi.force_collision(P, V, F, n, fc)
+    gi.sync()
+    Fg = F.download()
+    rows = []
+    for a in (0, fc.NFe // 2, fc.NFe, n // 2, n - 64):
+        a = max(0, min(a, n - 64))
+        F64, cond, near = O.force_collision_f64(pos, vel, fc, i_begin=a, i_end=a + 64)
+        e = np.linalg.norm(Fg[a : a + 64, :3].astype(np.float64) - F64[a : a + 64], axis=1)
+        ok = ~near[a : a + 64]
+        rows.append((np.max(e[ok] / cond[a : a + 64][ok]), np.max(e[ok] / np.linalg.norm(F64[a : a + 64][ok], axis=1))))
+    return {"rows_checked": 320, "cond_err_max": float(max(r[0] for r in rows)),
+            "rel_err_max": float(max(r[1] for r in rows))}
+
+
 def drift_leg(gi, steps_small: int, steps_large: int, check_every: int, steps_headline: int = 0) -> dict:
     """Conserved quantities over long runs.  N = 2,048 (1,536 + 512, 45 deg,
     1.2 v_esc): the GPU and the oracle's float restatement of the reference
@@ -218,7 +241,7 @@ def drift_leg(gi, steps_small: int, steps_large: int, check_every: int, steps_he
     n = len(pos)
     c0 = O.conserved_f64(pos, vel, fc)
     P, V, F = gi.DeviceArray(n, pos), gi.DeviceArray(n, vel), gi.DeviceArray(n, np.zeros((n, 4), np.float32))
-    gi.impact_steps(P, V, F, n, fc, mc, 1)  # plan + graph capture, outside the timing
+    gi.impact_steps(P, V, F, n, fc, mc, 8)  # plan + graph captures, outside the timing
     gi.sync()
     P.upload(pos)
     V.upload(vel)
@@ -240,6 +263,7 @@ def drift_leg(gi, steps_small: int, steps_large: int, check_every: int, steps_he
                                          target_spin=0.5)
     n = len(pos)
     P, V, F = gi.DeviceArray(n, pos), gi.DeviceArray(n, vel), gi.DeviceArray(n, np.zeros((n, 4), np.float32))
+    step1 = spot_parity(gi, pos, vel, fc, P, V, F)  # the first step's force against FP64
     c0 = gi.conserved_quantities(P, V, n, fc)
     series = []
     t0 = time.perf_counter()
@@ -248,7 +272,7 @@ def drift_leg(gi, steps_small: int, steps_large: int, check_every: int, steps_he
         c = gi.conserved_quantities(P, V, n, fc)
         series.append(rel(c0, c))
     wall = time.perf_counter() - t0
-    out["n65536"] = {"steps": steps_large, "check_every": check_every, "wall_s": wall,
+    out["n65536"] = {"steps": steps_large, "check_every": check_every, "wall_s": wall, "step1_force_parity": step1,
                      "final": series[-1],
                      "angmom_rel_change_max": max(x["angmom_rel_change"] for x in series),
                      "energy_rel_change_series": [x["energy_rel_change"] for x in series]}
@@ -274,14 +298,14 @@ def drift_leg(gi, steps_small: int, steps_large: int, check_every: int, steps_he
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n-target", type=int, default=196608)
     ap.add_argument("--n-impactor", type=int, default=65536)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--ensemble-sims", type=int, default=64, help="0 skips the ensemble leg")
-    ap.add_argument("--ensemble-steps", type=int, default=1000,
+    ap.add_argument("--ensemble-steps", type=int, default=5000,
                     help="batched steps of the ensemble leg (FP64 E / L every 100)")
     ap.add_argument("--only-ensemble", action="store_true", help="run the ensemble leg alone (profiling)")
     ap.add_argument("--serve-requests", type=int, default=24,
@@ -355,18 +379,7 @@ def main() -> int:
     P, V, F = gi.DeviceArray(n, pos), gi.DeviceArray(n, vel), gi.DeviceArray(n, np.zeros((n, 4), np.float32))
 
     # parity spot check on the initial state (rows spread over all materials)
-    from oracle import oracle as O
-
-    gi.force_collision(P, V, F, n, fc)
-    gi.sync()
-    Fg = F.download()
-    rows = []
-    for a in (0, fc.NFe // 2, fc.NFe, n // 2, n - 64):
-        F64, cond, near = O.force_collision_f64(pos, vel, fc, i_begin=a, i_end=a + 64)
-        e = np.linalg.norm(Fg[a : a + 64, :3].astype(np.float64) - F64[a : a + 64], axis=1)
-        ok = ~near[a : a + 64]
-        rows.append((np.max(e[ok] / cond[a : a + 64][ok]), np.max(e[ok] / np.linalg.norm(F64[a : a + 64][ok], axis=1))))
-    parity = {"rows_checked": 320, "cond_err_max": float(max(r[0] for r in rows)), "rel_err_max": float(max(r[1] for r in rows))}
+    parity = spot_parity(gi, pos, vel, fc, P, V, F)
     log(f"parity vs FP64 oracle: {parity}")
 
     for _ in range(args.warmup):

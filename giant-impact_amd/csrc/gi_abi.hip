@@ -12,6 +12,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <initializer_list>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -83,7 +84,7 @@ struct DevBuf {
 // HIP events around one timed launch; kind 0 = all-pairs gravity, 1 = move.
 // kind 2 = the whole all-pairs pass (for the mutual form: padding, the
 // all-pairs kernel and the per-particle sums).
-constexpr int kTimedKinds = 3;
+constexpr int kTimedKinds = 4;
 struct TimingPair {
     hipEvent_t a = nullptr, b = nullptr;
     int kind = 0;
@@ -108,6 +109,9 @@ struct Plan {
     long long iside_len = 0;  // floats
     long long jside_len = 0;  // floats
     long long chunk_len = 0;  // j per task of the mutual pass (0 if none)
+    // direct pass (tiling.variant == 3): one workgroup per 64 i-particles
+    bool direct = false;
+    std::vector<DirectBlock> dblocks;
 };
 
 // Scratch and plan of one owner (the drop-in entry points of a device, or
@@ -118,6 +122,7 @@ struct Workspace {
     DevBuf<float> iside, jside;    // mutual pass: row slots, j-side slabs
     DevBuf<MutualTask> mtasks;
     DevBuf<RowDesc> rows;
+    DevBuf<DirectBlock> dblocks;
     DevBuf<int> counts, starts, sorted, cellhash, block_sums;
     DevBuf<float4> spos, svel;  // positions / velocities in bucket order
     DevBuf<GravTask> tasks;
@@ -160,8 +165,8 @@ struct DevCtx {
     hipEvent_t ev_in = nullptr, ev_out = nullptr;
     std::vector<GraphEntry> graphs;
     std::vector<TimingPair> pending, pool;
-    long long timed_launches[kTimedKinds] = {0, 0, 0};
-    double timed_ms[kTimedKinds] = {0.0, 0.0, 0.0};
+    long long timed_launches[kTimedKinds] = {0, 0, 0, 0};
+    double timed_ms[kTimedKinds] = {0.0, 0.0, 0.0, 0.0};
 };
 
 static std::mutex g_ctx_mutex;
@@ -215,7 +220,7 @@ static bool same_tiling(const Tiling& a, const Tiling& b) {
 // too large for it takes the one-sided pass with the default task count.
 static long long sim_chunk_len(long long n) {
     Tiling t = choose_tiling(n, n * n, 1);
-    if (t.variant == 2) t = Tiling{1, 0, 0, 32768, 512};
+    if (t.variant >= 2) t = Tiling{1, 0, 0, 32768, 512};
     long long L = t.chunk_len;
     if (L <= 0) {
         const long long ib = 64LL * t.R;
@@ -341,13 +346,19 @@ static int build_plan(std::vector<SimDesc> sims, const Tiling& tl, bool batched,
         int rc = plan_mutual(sims, tl, batched, p);
         if (rc < 0) return rc;
     }
+    if (tl.variant == 3) {
+        for (size_t si = 0; si < sims.size(); ++si)
+            if (sims[si].active)
+                for (int i0 = sims[si].start; i0 < sims[si].end; i0 += 64) p.dblocks.push_back(DirectBlock{(int)si, i0});
+        p.direct = true;
+    }
     long long pbase = 0;
     for (size_t si = 0; si < sims.size(); ++si) {
         SimDesc& s = sims[si];
         const int n = s.end - s.start;
         s.partial_base = (int)pbase;
         s.nchunk = 0;
-        if (n == 0 || !s.active || s.gn > 0) continue;  // gn > 0: mutual pass
+        if (n == 0 || !s.active || s.gn > 0 || p.direct) continue;  // gn > 0: mutual pass
         const long long L = batched ? (tl.variant == 2 ? 16384 : tl.chunk_len) : sim_chunk_len(n);
         // j segments: iron [start, fe_end), silicate [fe_end, end); each split
         // into near-equal chunks of at most L.
@@ -442,6 +453,7 @@ static int ensure_plan(Workspace* ws, const std::vector<SimDesc>& sims, hipStrea
         if ((rc = upload(ws->mtasks, p.mtasks, st))) return rc;
         if ((rc = upload(ws->rows, p.rows, st))) return rc;
     }
+    if (p.direct && (rc = upload(ws->dblocks, p.dblocks, st))) return rc;
     if ((rc = ws->counts.ensure((size_t)p.buckets + 1))) return rc;
     if ((rc = ws->starts.ensure((size_t)p.buckets + 2))) return rc;
     if ((rc = ws->spos.ensure((size_t)p.ntot + 1))) return rc;
@@ -612,18 +624,62 @@ static int launch_gravity(DevCtx* ctx, Workspace* ws, const float4* pos, bool po
     return 0;
 }
 
-// Full force evaluation for the current plan: grid, all pairs, finish.
+// The direct pass of the current plan (force, or with pot the per-particle
+// potential into ws->pe).  Timed as both kind 0 and kind 2.
+static int launch_direct(DevCtx* ctx, Workspace* ws, const float4* pos, const float4* vel, float4* force, bool pot,
+                         hipStream_t st, bool timed) {
+    const Plan& p = ws->plan;
+    const int nb = (int)p.dblocks.size();
+    if (nb == 0) return 0;
+    TimingPair t0, t2;
+    if (timed) {
+        t2 = take_pair(ctx, 2);
+        t0 = take_pair(ctx, 0);
+        GI_HIP(hipEventRecord(t2.a, st));
+        GI_HIP(hipEventRecord(t0.a, st));
+    }
+    if (pot)
+        hipLaunchKernelGGL(k_direct<true>, dim3(nb), dim3(64 * kDirectWaves), 0, st, pos, vel, ws->dblocks.p,
+                           ws->sims.p, nullptr, ws->pe.p);
+    else
+        hipLaunchKernelGGL(k_direct<false>, dim3(nb), dim3(64 * kDirectWaves), 0, st, pos, vel, ws->dblocks.p,
+                           ws->sims.p, force, nullptr);
+    GI_HIP(hipGetLastError());
+    if (timed) {
+        GI_HIP(hipEventRecord(t0.b, st));
+        GI_HIP(hipEventRecord(t2.b, st));
+        ctx->pending.push_back(t0);
+        ctx->pending.push_back(t2);
+    }
+    return 0;
+}
+
+// Full force evaluation for the current plan: grid, all pairs, finish (or
+// the direct pass alone).
 static int run_force(DevCtx* ctx, Workspace* ws, const float4* pos, const float4* vel, float4* force,
                      hipStream_t st, bool timed) {
     const Plan& p = ws->plan;
     if (p.ntot == 0) return 0;
-    int rc = launch_grid(ws, pos, vel, st);
-    if (rc) return rc;
-    if ((rc = launch_gravity(ctx, ws, pos, false, st, timed))) return rc;
-    hipLaunchKernelGGL(k_finish, dim3(nblk(p.ntot, 256)), dim3(256), 0, st, force, p.ntot, ws->sims.p,
-                       (int)p.sims.size(), ws->partial.p, p.mutual ? ws->gsum.p : nullptr, ws->starts.p,
-                       ws->sorted.p, ws->spos.p, ws->svel.p, p.tiling.variant);
-    GI_HIP(hipGetLastError());
+    TimingPair whole;
+    if (timed) {
+        whole = take_pair(ctx, 3);
+        GI_HIP(hipEventRecord(whole.a, st));
+    }
+    int rc;
+    if (p.direct) {
+        if ((rc = launch_direct(ctx, ws, pos, vel, force, false, st, timed))) return rc;
+    } else {
+        if ((rc = launch_grid(ws, pos, vel, st))) return rc;
+        if ((rc = launch_gravity(ctx, ws, pos, false, st, timed))) return rc;
+        hipLaunchKernelGGL(k_finish, dim3(nblk(p.ntot, 256)), dim3(256), 0, st, force, p.ntot, ws->sims.p,
+                           (int)p.sims.size(), ws->partial.p, p.mutual ? ws->gsum.p : nullptr, ws->starts.p,
+                           ws->sorted.p, ws->spos.p, ws->svel.p, p.tiling.variant);
+        GI_HIP(hipGetLastError());
+    }
+    if (timed) {
+        GI_HIP(hipEventRecord(whole.b, st));
+        ctx->pending.push_back(whole);
+    }
     return 0;
 }
 
@@ -631,12 +687,16 @@ static int run_force(DevCtx* ctx, Workspace* ws, const float4* pos, const float4
 static int run_conserved(DevCtx* ctx, Workspace* ws, const float4* pos, const float4* vel, hipStream_t st) {
     const Plan& p = ws->plan;
     if (p.ntot == 0) return 0;
-    int rc = launch_grid(ws, pos, nullptr, st);
-    if (rc) return rc;
-    if ((rc = launch_gravity(ctx, ws, pos, true, st, false))) return rc;
-    hipLaunchKernelGGL(k_finish_potential, dim3(nblk(p.ntot, 256)), dim3(256), 0, st, p.ntot, ws->sims.p,
-                       (int)p.sims.size(), ws->partial.p, p.mutual ? ws->gsum.p : nullptr, ws->starts.p,
-                       ws->sorted.p, ws->spos.p, ws->pe.p);
+    int rc;
+    if (p.direct) {
+        if ((rc = launch_direct(ctx, ws, pos, vel, nullptr, true, st, false))) return rc;
+    } else {
+        if ((rc = launch_grid(ws, pos, nullptr, st))) return rc;
+        if ((rc = launch_gravity(ctx, ws, pos, true, st, false))) return rc;
+        hipLaunchKernelGGL(k_finish_potential, dim3(nblk(p.ntot, 256)), dim3(256), 0, st, p.ntot, ws->sims.p,
+                           (int)p.sims.size(), ws->partial.p, p.mutual ? ws->gsum.p : nullptr, ws->starts.p,
+                           ws->sorted.p, ws->spos.p, ws->pe.p);
+    }
     const int nb = (int)p.block_sim.size();
     hipLaunchKernelGGL(k_moments, dim3(nb), dim3(256), 0, st, pos, vel, (const double*)ws->pe.p,
                        ws->block_sim.p, ws->block_first.p, ws->sims.p, 0.0, 0.0, ws->moments.p);
@@ -701,13 +761,24 @@ static void key_append(std::vector<unsigned char>& k, const T& v) {
     k.insert(k.end(), b, b + sizeof(T));
 }
 
-// Capture `body` on a private stream into a graph cached under `key`.
+// Capture `body` on a private stream into a graph cached under `key`.  The
+// cache keeps the kGraphCacheSize most recently used graphs; `keep` lists
+// graphs the caller already holds, which are never evicted (a caller that
+// needs several graphs at once asks for them one by one).  An evicted graph
+// may still have launches in flight from an earlier call on any stream, so
+// the device is synchronised before it is destroyed.
+constexpr size_t kGraphCacheSize = 8;
+
 template <typename F>
 static int get_graph(DevCtx* ctx, std::vector<GraphEntry>& cache, const std::vector<unsigned char>& key,
-                     unsigned long long generation, F body, hipGraphExec_t* out) {
-    for (auto& g : cache)
-        if (g.key == key && g.generation == generation) {
-            *out = g.exec;
+                     unsigned long long generation, F body, hipGraphExec_t* out,
+                     std::initializer_list<hipGraphExec_t> keep = {}) {
+    for (size_t k = 0; k < cache.size(); ++k)
+        if (cache[k].key == key && cache[k].generation == generation) {
+            GraphEntry hit = cache[k];  // most recently used goes last
+            cache.erase(cache.begin() + (long)k);
+            cache.push_back(hit);
+            *out = hit.exec;
             return 0;
         }
     if (!ctx->capture_stream) GI_HIP(hipStreamCreateWithFlags(&ctx->capture_stream, hipStreamNonBlocking));
@@ -729,12 +800,32 @@ static int get_graph(DevCtx* ctx, std::vector<GraphEntry>& cache, const std::vec
         (void)hipGraphDestroy(g);
         return set_error((int)ie, "hipGraphInstantiate: %s", hipGetErrorString(ie));
     }
-    if (cache.size() >= 8) {
-        cache.front().destroy();
-        cache.erase(cache.begin());
+    if (cache.size() >= kGraphCacheSize) {
+        for (size_t k = 0; k < cache.size(); ++k) {
+            bool held = false;
+            for (hipGraphExec_t h : keep) held = held || cache[k].exec == h;
+            if (held) continue;
+            GI_HIP(hipDeviceSynchronize());
+            cache[k].destroy();
+            cache.erase(cache.begin() + (long)k);
+            break;
+        }
     }
     cache.push_back(e);
     *out = e.exec;
+    return 0;
+}
+
+// Steps per graph of the multi-step form: a whole graph launch per step is
+// a fixed cost that small simulations feel.
+constexpr int kStepsPerGraph = 8;
+
+// Launch `count` steps: the kStepsPerGraph-step graph as often as it fits,
+// then the one-step graph.
+static int launch_step_run(hipGraphExec_t one, hipGraphExec_t many, int count, hipStream_t st) {
+    if (many)
+        for (; count >= kStepsPerGraph; count -= kStepsPerGraph) GI_HIP(hipGraphLaunch(many, st));
+    for (; count > 0; --count) GI_HIP(hipGraphLaunch(one, st));
     return 0;
 }
 
@@ -782,7 +873,12 @@ struct gi_ensemble {
     hipStream_t stream = nullptr;
     std::vector<gi::GraphEntry> graphs;
     ~gi_ensemble() {
+        // launches of its graphs may still be in flight on its device
+        int cur = -1;
+        if (hipGetDevice(&cur) == hipSuccess && device >= 0 && cur != device) (void)hipSetDevice(device);
+        (void)hipDeviceSynchronize();
         for (auto& g : graphs) g.destroy();
+        if (cur >= 0 && cur != device) (void)hipSetDevice(cur);
     }
 };
 
@@ -891,9 +987,10 @@ int gi_plan_info(int* mutual, int* r, long long* pair_tasks, long long* chunk_le
     if (rc) return rc;
     const Plan& p = ctx->ws.plan;
     const bool valid = ctx->ws.plan_valid;
-    if (mutual) *mutual = valid && p.mutual ? 1 : 0;
+    if (mutual) *mutual = !valid ? 0 : (p.direct ? 2 : (p.mutual ? 1 : 0));
     if (r) *r = valid ? p.tiling.R : 0;
-    if (pair_tasks) *pair_tasks = valid ? (long long)(p.tasks.size() + p.mtasks.size()) : 0;
+    if (pair_tasks)
+        *pair_tasks = !valid ? 0 : (p.direct ? (long long)p.dblocks.size() : (long long)(p.tasks.size() + p.mtasks.size()));
     if (chunk_len) *chunk_len = valid ? p.chunk_len : 0;
     return 0;
 }
@@ -929,7 +1026,8 @@ int gi_set_tiling(int r, int variant, long long chunk_len, long long target_task
     }
     if (r != 1 && r != 2 && r != 4 && r != 8)
         return set_error(GI_E_INVALID_ARG, "gi_set_tiling: r must be 1, 2, 4 or 8");
-    if (variant < 0 || variant > 2) return set_error(GI_E_INVALID_ARG, "gi_set_tiling: variant must be 0, 1 or 2");
+    if (variant < 0 || variant > 3)
+        return set_error(GI_E_INVALID_ARG, "gi_set_tiling: variant must be 0, 1, 2 or 3");
     if (chunk_len < 0 || target_tasks < 0 || min_chunk < 0)
         return set_error(GI_E_INVALID_ARG, "gi_set_tiling: negative size");
     tiling_override().active = 1;
@@ -1206,18 +1304,25 @@ int gi_impact_steps(gi_float4* pos, gi_float4* vel, gi_float4* force, int n, con
     key_append(key, n);
     key_append(key, *fc);
     key_append(key, m);
-    hipGraphExec_t step;
-    rc = get_graph(ctx, ctx->graphs, key, ctx->ws.generation,
-                   [&](hipStream_t cs) -> int {
-                       int r = run_force(ctx, &ctx->ws, P, V, F, cs, false);
-                       if (r) return r;
-                       hipLaunchKernelGGL(k_move_collision, dim3(nblk(n, 256)), dim3(256), 0, cs, P, V,
-                                          (const float4*)F, n, m.Dt, m.DtOverMassFe, m.DtOverMassSi, m.NFe);
-                       GI_HIP(hipGetLastError());
-                       return 0;
-                   },
-                   &step);
+    auto body = [&](hipStream_t cs, int steps) -> int {
+        for (int k = 0; k < steps; ++k) {
+            int r = run_force(ctx, &ctx->ws, P, V, F, cs, false);
+            if (r) return r;
+            hipLaunchKernelGGL(k_move_collision, dim3(nblk(n, 256)), dim3(256), 0, cs, P, V, (const float4*)F, n,
+                               m.Dt, m.DtOverMassFe, m.DtOverMassSi, m.NFe);
+            GI_HIP(hipGetLastError());
+        }
+        return 0;
+    };
+    hipGraphExec_t step, steps = nullptr;
+    rc = get_graph(ctx, ctx->graphs, key, ctx->ws.generation, [&](hipStream_t cs) { return body(cs, 1); }, &step);
     if (rc) return rc;
+    if (n_steps >= kStepsPerGraph && (recenter_rate == 0 || recenter_rate >= kStepsPerGraph)) {
+        key_append(key, kStepsPerGraph);
+        rc = get_graph(ctx, ctx->graphs, key, ctx->ws.generation,
+                       [&](hipStream_t cs) { return body(cs, kStepsPerGraph); }, &steps, {step});
+        if (rc) return rc;
+    }
     hipGraphExec_t rec = nullptr;
     if (recenter_rate > 0) {
         // The recenter uses the move's masses: m = Dt / (Dt/m).
@@ -1240,11 +1345,15 @@ int gi_impact_steps(gi_float4* pos, gi_float4* vel, gi_float4* force, int n, con
                        [&](hipStream_t cs) -> int {
                            return launch_recenter(ctx, P, V, n, fc->NFe, mfe, msi, target_com3, target_vel3, cs);
                        },
-                       &rec);
+                       &rec, {step, steps});
         if (rc) return rc;
     }
-    for (int s = 1; s <= n_steps; ++s) {
-        GI_HIP(hipGraphLaunch(step, br.run));
+    // as steps 1..n_steps with a recenter after every step s with
+    // s % recenter_rate == 0
+    for (int s = 0; s < n_steps;) {
+        const int k = rec ? std::min(n_steps - s, recenter_rate - s % recenter_rate) : n_steps - s;
+        if ((rc = launch_step_run(step, steps, k, br.run))) return rc;
+        s += k;
         if (rec && s % recenter_rate == 0) GI_HIP(hipGraphLaunch(rec, br.run));
     }
     return br.end();
@@ -1282,20 +1391,27 @@ int gi_separate_steps(gi_float4* pos, gi_float4* vel, gi_float4* force, int n, c
     key_append(key, m);
     key_append(key, damp_rate_body1);
     key_append(key, damp_rate_body2);
-    hipGraphExec_t step;
-    rc = get_graph(ctx, ctx->graphs, key, ctx->ws.generation,
-                   [&](hipStream_t cs) -> int {
-                       int r = run_force(ctx, &ctx->ws, P, V, F, cs, false);
-                       if (r) return r;
-                       hipLaunchKernelGGL(k_move_damped_separate, dim3(nblk(n, 256)), dim3(256), 0, cs, P, V,
-                                          (const float4*)F, n, m.Dt, m.DtOverMassFe, m.DtOverMassSi, m.boarder1,
-                                          m.boarder2, m.boarder3, damp_rate_body1, damp_rate_body2);
-                       GI_HIP(hipGetLastError());
-                       return 0;
-                   },
-                   &step);
+    auto body = [&](hipStream_t cs, int steps) -> int {
+        for (int k = 0; k < steps; ++k) {
+            int r = run_force(ctx, &ctx->ws, P, V, F, cs, false);
+            if (r) return r;
+            hipLaunchKernelGGL(k_move_damped_separate, dim3(nblk(n, 256)), dim3(256), 0, cs, P, V,
+                               (const float4*)F, n, m.Dt, m.DtOverMassFe, m.DtOverMassSi, m.boarder1, m.boarder2,
+                               m.boarder3, damp_rate_body1, damp_rate_body2);
+            GI_HIP(hipGetLastError());
+        }
+        return 0;
+    };
+    hipGraphExec_t step, steps = nullptr;
+    rc = get_graph(ctx, ctx->graphs, key, ctx->ws.generation, [&](hipStream_t cs) { return body(cs, 1); }, &step);
     if (rc) return rc;
-    for (int s = 0; s < n_steps; ++s) GI_HIP(hipGraphLaunch(step, br.run));
+    if (n_steps >= kStepsPerGraph) {
+        key_append(key, kStepsPerGraph);
+        rc = get_graph(ctx, ctx->graphs, key, ctx->ws.generation,
+                       [&](hipStream_t cs) { return body(cs, kStepsPerGraph); }, &steps, {step});
+        if (rc) return rc;
+    }
+    if ((rc = launch_step_run(step, steps, n_steps, br.run))) return rc;
     return br.end();
 }
 
@@ -1431,8 +1547,11 @@ int gi_kernel_timing(const char* kernel, long long* launches, double* total_ms) 
         kind = 1;
     else if (kernel && std::strcmp(kernel, "gravity_pass") == 0)
         kind = 2;
+    else if (kernel && std::strcmp(kernel, "force") == 0)
+        kind = 3;
     else if (kernel && std::strcmp(kernel, "gravity") != 0)
-        return set_error(GI_E_INVALID_ARG, "unknown kernel '%s' (timed: gravity, gravity_pass, move)", kernel);
+        return set_error(GI_E_INVALID_ARG, "unknown kernel '%s' (timed: gravity, gravity_pass, force, move)",
+                         kernel);
     DevCtx* ctx;
     int rc = current_ctx(&ctx);
     if (rc) return rc;

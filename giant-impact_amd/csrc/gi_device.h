@@ -44,6 +44,12 @@ struct MutualTask {
     int sim, row, j0, j1, slot, jfe, diag, pad;
 };
 
+// One workgroup of the direct pass: 64 i-particles [i0, i0 + 64) of
+// simulation sim (the last block of a simulation may be ragged).
+struct DirectBlock {
+    int sim, i0;
+};
+
 // The i-side slots of one row: slot0 .. slot0 + nslot - 1, in task order.
 struct RowDesc {
     int slot0, nslot;

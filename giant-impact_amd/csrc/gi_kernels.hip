@@ -595,6 +595,185 @@ __global__ __launch_bounds__(256) void k_finish(float4* __restrict__ force, int 
 }
 
 // ---------------------------------------------------------------------------
+// Direct pass: the whole force of a small simulation in one launch
+// ---------------------------------------------------------------------------
+//
+// For small N the three passes above are bound by their dozen launches, not by
+// arithmetic.  k_direct computes the same force in one: a workgroup holds 64
+// i-particles (one per lane) and its kDirectWaves wavefronts share the
+// simulation's j-range, each segment (iron j, then silicate j, so the gravity
+// product is one factor per segment) cut into groups of four j dealt out in
+// turn (wavefront w takes groups w, w + kDirectWaves, ...).  Dealing keeps
+// the contact work -- concentrated on the j near the block's own particles --
+// spread over all wavefronts.  Every pair takes the all-pairs pass's
+// branch-free gravity term (j through wave-uniform scalar loads; float tiles
+// of kDirectTile j added in FP64); when some lane of a group has a pair with
+// r^2 < 1 the group also adds, for its pairs in contact, the exact contact
+// term minus that gravity term, with the arithmetic of k_finish.  The
+// wavefronts' sums are added in FP64 in a fixed order, so the result depends
+// on the simulation alone.  POT: the per-particle potential of
+// k_finish_potential instead of the force.
+constexpr int kDirectWaves = 16;
+constexpr int kDirectTile = 256;  // j per float tile
+// r^2 (with FMAs and kSelfEps) of every pair whose exactly rounded r is below
+// 1 lies below this: the margin covers the few ulps between the two forms.
+constexpr float kDirectContactR2 = 1.0001f;
+
+template <bool POT>
+__global__ __launch_bounds__(64 * kDirectWaves) void k_direct(const float4* __restrict__ pos,
+                                                              const float4* __restrict__ vel,
+                                                              const DirectBlock* __restrict__ blocks,
+                                                              const SimDesc* __restrict__ sims,
+                                                              float4* __restrict__ force,
+                                                              double* __restrict__ pe) {
+    __shared__ double red[3][kDirectWaves][64];
+    const int lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const DirectBlock b = blocks[blockIdx.x];
+    const int start = sims[b.sim].start, fe_end = sims[b.sim].fe_end, end = sims[b.sim].end;
+    const PairLaw law = sims[b.sim].law;  // in registers: the contact branch reads it often
+    const int i = b.i0 + lane;
+    const bool live = i < end;
+    const int ic = live ? i : end - 1;
+    const float4 pi = pos[ic];
+    const bool ife = ic < fe_end;
+
+    float4 vi = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if constexpr (!POT) vi = vel[ic];
+    double sx = 0.0, sy = 0.0, sz = 0.0;  // this wavefront's gravity plus contact corrections
+    for (int seg = 0; seg < 2; ++seg) {
+        const bool jfe = seg == 0;  // iron j first, then silicate j
+        const int a = jfe ? start : fe_end;
+        const int e = jfe ? fe_end : end;
+        const int ngroups = (e - a + 3) / 4;
+        if (wv >= ngroups) continue;
+        const int test = (int)ife + (int)jfe;
+        const float g = gravity_product(law, test);
+        double ux = 0.0, uy = 0.0, uz = 0.0;  // float tiles of kDirectTile j, folded in FP64
+        double cx = 0.0, cy = 0.0, cz = 0.0;
+        // The exact contact term of pair (i, j) minus the gravity term s*dp
+        // the tile sum took for it (k_finish's, resp. k_finish_potential's,
+        // test and arithmetic).
+        auto contact = [&](int j, const float4 pj, float s) {
+            const float dx = pj.x - pi.x;
+            const float dy = pj.y - pi.y;
+            const float dz = pj.z - pi.z;
+            if constexpr (!POT) {
+                const float r2 = dx * dx + dy * dy + dz * dz;
+                const float r = sqrtf(r2);
+                if (r < 1.0f && j != i) {
+                    const float invr = 1.0f / r;
+                    const float4 vj = vel[j];
+                    const float dvx = vj.x - vi.x, dvy = vj.y - vi.y, dvz = vj.z - vi.z;
+                    const float dot = dx * dvx + dy * dvy + dz * dvz;
+                    const float fm = contact_pair_mag(law, test, r, r2, dot);
+                    const float sf = g * s;
+                    cx += (double)(fm * dx * invr) - (double)(sf * dx);
+                    cy += (double)(fm * dy * invr) - (double)(sf * dy);
+                    cz += (double)(fm * dz * invr) - (double)(sf * dz);
+                }
+            } else {
+                const double ddx = (double)pj.x - pi.x, ddy = (double)pj.y - pi.y, ddz = (double)pj.z - pi.z;
+                const double r = sqrt(ddx * ddx + ddy * ddy + ddz * ddz);
+                if (r < 1.0 && j != i) {
+                    const double G = gravity_product(law, test);
+                    const double K = approach_k(law, test);
+                    const double exact = G * (r - 2.0) + (K / 3.0) * (r - 1.0) * (r - 1.0) * (r + 2.0);
+                    cx += exact + G * (double)s;
+                }
+            }
+        };
+        // this wavefront's groups, kDirectTile / 4 of them per float tile
+        for (int g0 = wv; g0 < ngroups; g0 += kDirectWaves * (kDirectTile / 4)) {
+            const int g1 = min(g0 + kDirectWaves * (kDirectTile / 4), ngroups);
+            float tx = 0.0f, ty = 0.0f, tz = 0.0f;
+            auto pair = [&](const float4 q, float& s, bool& near) {
+                const float dx = q.x - pi.x;
+                const float dy = q.y - pi.y;
+                const float dz = q.z - pi.z;
+                if constexpr (!POT) {
+                    const float r2f = fast_r2(dx, dy, dz);
+                    s = fast_inv_r3(r2f);
+                    tx = __builtin_fmaf(s, dx, tx);
+                    ty = __builtin_fmaf(s, dy, ty);
+                    tz = __builtin_fmaf(s, dz, tz);
+                    near |= r2f < kDirectContactR2;
+                } else {
+                    const float r2f = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+                    s = r2f > 0.0f ? __builtin_amdgcn_rsqf(r2f) : 0.0f;
+                    tx += s;
+                    near |= r2f < kDirectContactR2;
+                }
+            };
+            // The next group's loads are in flight while this one is computed
+            // (indices clamped into the segment; a partial last group only
+            // takes its first e - j pairs).
+            float4 q[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) q[u] = pos[min(a + 4 * g0 + u, e - 1)];
+            for (int grp = g0; grp < g1; grp += kDirectWaves) {
+                const int j = a + 4 * grp;
+                const int jn = grp + kDirectWaves < g1 ? j + 4 * kDirectWaves : j;
+                float4 nq[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) nq[u] = pos[min(jn + u, e - 1)];
+                float s[4];
+                bool near = false;
+                if (j + 4 <= e) {
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) pair(q[u], s[u], near);
+                    if (near) {
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) contact(j + u, q[u], s[u]);
+                    }
+                } else {
+#pragma unroll
+                    for (int u = 0; u < 4; ++u)
+                        if (j + u < e) pair(q[u], s[u], near);
+                    if (near) {
+#pragma unroll
+                        for (int u = 0; u < 4; ++u)
+                            if (j + u < e) contact(j + u, q[u], s[u]);
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) q[u] = nq[u];
+            }
+            ux += (double)tx;
+            uy += (double)ty;
+            uz += (double)tz;
+        }
+        if constexpr (!POT) {
+            sx += (double)g * ux + cx;
+            sy += (double)g * uy + cy;
+            sz += (double)g * uz + cz;
+        } else {
+            sx += -(double)g * ux + cx;
+        }
+    }
+    red[0][wv][lane] = sx;
+    red[1][wv][lane] = sy;
+    red[2][wv][lane] = sz;
+    __syncthreads();
+    if (wv == 0 && live) {
+        double fx = 0.0, fy = 0.0, fz = 0.0;
+#pragma unroll
+        for (int w = 0; w < kDirectWaves; ++w) {
+            fx += red[0][w][lane];
+            fy += red[1][w][lane];
+            fz += red[2][w][lane];
+        }
+        if constexpr (!POT) {
+            force[i].x = (float)fx;
+            force[i].y = (float)fy;
+            force[i].z = (float)fz;
+        } else {
+            pe[i] = 0.5 * fx;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Move kernels
 // ---------------------------------------------------------------------------
 
@@ -897,6 +1076,10 @@ GI_GRAVITY_INST(8, true, 0)
 #undef GI_GRAVITY_INST
 #define GI_MUTUAL_INST(R, POT) \
     template __global__ void k_mutual<R, POT>(const float4*, const MutualTask*, int, const SimDesc*, float*, float*);
+template __global__ void k_direct<false>(const float4*, const float4*, const DirectBlock*, const SimDesc*, float4*,
+                                        double*);
+template __global__ void k_direct<true>(const float4*, const float4*, const DirectBlock*, const SimDesc*, float4*,
+                                       double*);
 GI_MUTUAL_INST(1, false)
 GI_MUTUAL_INST(2, false)
 GI_MUTUAL_INST(4, false)

@@ -3,7 +3,9 @@
 // R            i-particles held in registers per lane (a wavefront covers 64*R)
 // variant      0 = one-sided pass, r^-3 as rsq cubed; 1 = one-sided, rsq *
 //              rcp (1 multiply, 2 transcendental ops); 2 = mutual pass (each
-//              pair once for both particles, rows of 64*R, rsq cubed)
+//              pair once for both particles, rows of 64*R, rsq cubed);
+//              3 = direct pass (small simulations: the whole force in one
+//              launch, 64 i-particles per workgroup; R and the sizes unused)
 // chunk_len    j-particles per task (0 = derive from target_tasks)
 // target_tasks wavefront tasks to aim for (several per SIMD of the 256 CUs x
 //              4 SIMDs so the tail of the launch stays short)
@@ -73,8 +75,9 @@ inline const std::vector<TuningRow>& tuning_table() {
             TuningRow r;
             if (std::sscanf(line, "%lld %d %d %lld %lld %lld", &r.n_max, &r.t.R, &r.t.variant, &r.t.chunk_len,
                             &r.t.target_tasks, &r.t.min_chunk) == 6 &&
-                (r.t.R == 1 || r.t.R == 2 || r.t.R == 4 || r.t.R == 8) && r.t.variant >= 0 && r.t.variant <= 2 &&
-                r.t.chunk_len >= 0 && r.t.target_tasks >= 0 && (r.t.target_tasks > 0 || r.t.chunk_len > 0))
+                (r.t.R == 1 || r.t.R == 2 || r.t.R == 4 || r.t.R == 8) && r.t.variant >= 0 && r.t.variant <= 3 &&
+                r.t.chunk_len >= 0 && r.t.target_tasks >= 0 &&
+                (r.t.target_tasks > 0 || r.t.chunk_len > 0 || r.t.variant == 3))
                 rows.push_back(r);
         }
         std::fclose(f);

@@ -476,8 +476,8 @@ def plan_info() -> dict:
     """The plan of the last drop-in force call on the current device."""
     m, r, t, L = C.c_int(), C.c_int(), C.c_longlong(), C.c_longlong()
     call("gi_plan_info", C.byref(m), C.byref(r), C.byref(t), C.byref(L))
-    return {"mutual": bool(m.value), "r": r.value, "pair_tasks": t.value, "chunk_len": L.value,
-            "kernel": "k_mutual" if m.value else "k_gravity"}
+    return {"mutual": m.value == 1, "direct": m.value == 2, "r": r.value, "pair_tasks": t.value,
+            "chunk_len": L.value, "kernel": ("k_gravity", "k_mutual", "k_direct")[m.value]}
 
 
 def sync() -> None:

@@ -1,11 +1,13 @@
 """Autotuner of the all-pairs pass (run on the GPU).
 
-For each size class it times the whole all-pairs pass (HIP events, via
-gi_kernel_timing("gravity_pass")) over a grid of tilings -- the one-sided
-pass with R (i-particles per lane), the r^-3 variant and the number of
-wavefront tasks, and the mutual pass (variant 2) with R (rows of 64*R) and
-its chunk length -- and writes the fastest to tuning/gfx950.txt, which
-libgiant_impact.so reads on first use:
+For each size class it times the whole force evaluation (HIP events, via
+gi_kernel_timing("force"): contact grid, all-pairs pass and contact
+correction, or the direct pass alone) over a grid of tilings -- the
+one-sided pass with R (i-particles per lane), the r^-3 variant and the
+number of wavefront tasks, the mutual pass (variant 2) with R (rows of 64*R)
+and its chunk length, and for the small classes the direct pass (variant 3)
+-- and writes the fastest to tuning/gfx950.txt, which libgiant_impact.so
+reads on first use:
 
     n_max R variant chunk_len target_tasks min_chunk
 
@@ -36,6 +38,11 @@ CLASSES = [
 ]
 
 
+# classes up to this size also try the direct pass (one launch per force;
+# it wins where the three-pass form is bound by its launches)
+DIRECT_MAX_CLASS = 24576
+
+
 def time_gravity(n_target: int, n_impactor: int, reps: int):
     pos, vel, fc, _, _ = make_impact(n_target, n_impactor, seed=77)
     n = len(pos)
@@ -52,7 +59,7 @@ def time_gravity(n_target: int, n_impactor: int, reps: int):
         sync()
         lib.gi_set_kernel_timing(0)
         L, T = C.c_longlong(), C.c_double()
-        call("gi_kernel_timing", b"gravity_pass", C.byref(L), C.byref(T))
+        call("gi_kernel_timing", b"force", C.byref(L), C.byref(T))
         return T.value / max(L.value, 1)
 
     return run, n
@@ -131,6 +138,8 @@ def main(argv=None) -> int:
         # (R, variant, chunk_len, target_tasks, min_chunk)
         cands = [(R, v, 0, tt, mc) for R, v, tt in itertools.product(Rs, variants, targets)]
         cands += [(R, 2, L, 0, 64) for R, L in itertools.product([4, 8], mutual_Ls)]
+        if n_max <= DIRECT_MAX_CLASS:
+            cands.append((1, 3, 0, 0, 0))
         best = None
         for cand in cands:
             call("gi_set_tiling", *cand)

@@ -158,10 +158,11 @@ int gi_sync(void);
 /* Free and total device memory in bytes (hipMemGetInfo). */
 int gi_mem_info(size_t *free_bytes, size_t *total_bytes);
 /* The plan of the last drop-in force call on the current device: mutual = 1
- * when it uses the mutual all-pairs pass (k_mutual) for some simulation, r =
- * i-particles per lane, pair_tasks = wavefront tasks of the all-pairs
- * kernel(s), chunk_len = j per task of the mutual pass.  Zeros before the
- * first call.  Any pointer may be NULL. */
+ * when it uses the mutual all-pairs pass (k_mutual) for some simulation, 2
+ * when it uses the direct pass (k_direct), 0 for the one-sided pass
+ * (k_gravity); r = i-particles per lane, pair_tasks = wavefront tasks of the
+ * all-pairs kernel(s) (workgroups of the direct pass), chunk_len = j per task
+ * of the mutual pass.  Zeros before the first call.  Any pointer may be NULL. */
 int gi_plan_info(int *mutual, int *r, long long *pair_tasks, long long *chunk_len);
 
 /* ---- the four step-loop kernels ----------------------------------------- */
@@ -293,7 +294,8 @@ int gi_find_earth_moon(const gi_float4 *pos, const gi_float4 *vel, int n, int nf
  * the library's lib/ directory, written by the autotuner): r = i-particles per
  * lane (1, 2, 4 or 8), variant = the all-pairs pass (0: one-sided, r^-3 as rsq
  * cubed; 1: one-sided, rsq*rcp; 2: mutual -- each pair evaluated once for both
- * particles, rows of 64*r particles), chunk_len = j-particles per task (0 =
+ * particles, rows of 64*r particles; 3: direct -- small simulations, the whole
+ * force, contacts included, in one launch), chunk_len = j-particles per task (0 =
  * derived from target_tasks),
  * target_tasks = wavefront tasks to aim for, min_chunk = lower bound of
  * chunk_len.  r = 0 restores the table.  Plans built with another tiling are
@@ -328,9 +330,10 @@ int gi_set_dispatch_limit(long long max_tasks);
 /* ---- kernel timing (HIP events around the all-pairs force kernel) ------- */
 
 /* Enable/disable per-launch timing (HIP events on the launching stream) of the
- * all-pairs gravity kernel ("gravity") and of gi_move_collision ("move"); when on,
- * gi_kernel_timing returns the launch count and the summed milliseconds since
- * the last reset. */
+ * all-pairs kernel(s) ("gravity"), the whole all-pairs pass ("gravity_pass"),
+ * the whole force evaluation, contacts included ("force"), and of
+ * gi_move_collision ("move"); when on, gi_kernel_timing returns the launch
+ * count and the summed milliseconds since the last reset. */
 int gi_set_kernel_timing(int enable);
 int gi_kernel_timing(const char *kernel, long long *launches, double *total_ms);
 int gi_reset_kernel_timing(void);

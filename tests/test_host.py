@@ -137,7 +137,7 @@ def test_invalid_arguments_are_rejected_before_touching_a_device(gi):
     assert lib.gi_force_separate(vp(1), vp(1), vp(1), 10, C.byref(fs), None) == -1
     assert lib.gi_move_collision(vp(1), vp(1), vp(1), -3, C.byref(mc), None) == -1
     assert lib.gi_check_block_multiple(1000) == -2 and lib.gi_check_block_multiple(1024) == 0
-    assert lib.gi_set_tiling(3, 0, 0, 0, 0) == -1 and lib.gi_set_tiling(4, 3, 0, 0, 0) == -1
+    assert lib.gi_set_tiling(3, 0, 0, 0, 0) == -1 and lib.gi_set_tiling(4, 4, 0, 0, 0) == -1
     assert lib.gi_set_tiling(0, 0, 0, 0, 0) == 0
     assert lib.gi_set_dispatch_limit(-1) == -1 and lib.gi_set_dispatch_limit(0) == 0
     assert lib.gi_impact_steps(vp(1), vp(1), vp(1), 10, C.byref(fc), C.byref(mc), 5, 1, None, None, None) == -1
@@ -225,5 +225,7 @@ def test_every_tuning_row_is_accepted(gi):
     gi.call("gi_set_tiling", 8, 2, 512, 0, 64)
     try:
         assert gi.tiling_for(1000)["variant"] == 2
+        gi.call("gi_set_tiling", 1, 3, 0, 0, 0)
+        assert gi.tiling_for(1000)["variant"] == 3
     finally:
         gi.call("gi_set_tiling", 0, 0, 0, 0, 0)
