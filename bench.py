@@ -196,6 +196,32 @@ def serving_leg(gi, n_requests: int, mean_gap_s: float, seed: int = 20261206) ->
     }
 
 
+def small_leg(gi, steps: int) -> dict:
+    """The N = 2,048 impact (1,536 + 512, 45 deg, 1.2 v_esc) stepped through
+    the drop-in calls, one force and one move launch per step (no graphs), with
+    the force evaluation timed by HIP events: the small-simulation pass the
+    tuning table picks at this size, for a kernel trace."""
+    pos, vel, fc, mc, _ = gi.make_impact(1536, 512, seed=20261202, angle_deg=45.0, speed_vesc=1.2)
+    n = len(pos)
+    P, V, F = gi.DeviceArray(n, pos), gi.DeviceArray(n, vel), gi.DeviceArray(n, np.zeros((n, 4), np.float32))
+    gi.force_collision(P, V, F, n, fc)  # plan
+    gi.sync()
+    lib = gi.load()
+    lib.gi_set_kernel_timing(1)
+    lib.gi_reset_kernel_timing()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        gi.force_collision(P, V, F, n, fc)
+        gi.move_collision(P, V, F, n, mc)
+    gi.sync()
+    wall = time.perf_counter() - t0
+    lib.gi_set_kernel_timing(0)
+    L, T = C.c_longlong(), C.c_double()
+    gi.call("gi_kernel_timing", b"force", C.byref(L), C.byref(T))
+    return {"n": n, "steps": steps, "kernel": gi.plan_info()["kernel"], "wall_us_per_step": wall / steps * 1e6,
+            "force_us": T.value / max(L.value, 1) * 1e3, "pair_per_s": float(n) * n * steps / wall}
+
+
 def spot_parity(gi, pos, vel, fc, P, V, F) -> dict:
     """One force evaluation on the GPU against the FP64 oracle on 5 x 64 rows
     spread over the materials and bodies (iron start and middle, the Fe/Si
@@ -308,6 +334,8 @@ def main() -> int:
     ap.add_argument("--ensemble-steps", type=int, default=5000,
                     help="batched steps of the ensemble leg (FP64 E / L every 100)")
     ap.add_argument("--only-ensemble", action="store_true", help="run the ensemble leg alone (profiling)")
+    ap.add_argument("--only-small", action="store_true",
+                    help="run the N=2,048 impact alone through the drop-in calls (profiling the small-N pass)")
     ap.add_argument("--serve-requests", type=int, default=24,
                     help="requests of the serving leg (0 = skip)")
     ap.add_argument("--serve-gap", type=float, default=1.5, help="mean Poisson inter-arrival gap (s)")
@@ -340,6 +368,10 @@ def main() -> int:
     if args.only_ensemble:
         out = ensemble_leg(gi, args.ensemble_sims, args.ensemble_steps)
         print(json.dumps({"ensemble": out}), flush=True)
+        return 0
+
+    if args.only_small:
+        print(json.dumps({"small": small_leg(gi, 200)}), flush=True)
         return 0
 
     if args.profile_probe:

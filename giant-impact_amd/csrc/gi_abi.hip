@@ -109,8 +109,10 @@ struct Plan {
     long long iside_len = 0;  // floats
     long long jside_len = 0;  // floats
     long long chunk_len = 0;  // j per task of the mutual pass (0 if none)
-    // direct pass (tiling.variant == 3): one workgroup per 64 i-particles
+    // direct pass (tiling.variant == 3): one workgroup per 64 / direct_lanes
+    // i-particles
     bool direct = false;
+    int direct_lanes = 1;
     std::vector<DirectBlock> dblocks;
 };
 
@@ -347,9 +349,12 @@ static int build_plan(std::vector<SimDesc> sims, const Tiling& tl, bool batched,
         if (rc < 0) return rc;
     }
     if (tl.variant == 3) {
+        // R = lanes per i-particle (j-streams per wavefront): 1, 2 or 4
+        p.direct_lanes = std::min(tl.R, 4);
+        const int ib = 64 / p.direct_lanes;
         for (size_t si = 0; si < sims.size(); ++si)
             if (sims[si].active)
-                for (int i0 = sims[si].start; i0 < sims[si].end; i0 += 64) p.dblocks.push_back(DirectBlock{(int)si, i0});
+                for (int i0 = sims[si].start; i0 < sims[si].end; i0 += ib) p.dblocks.push_back(DirectBlock{(int)si, i0});
         p.direct = true;
     }
     long long pbase = 0;
@@ -638,12 +643,23 @@ static int launch_direct(DevCtx* ctx, Workspace* ws, const float4* pos, const fl
         GI_HIP(hipEventRecord(t2.a, st));
         GI_HIP(hipEventRecord(t0.a, st));
     }
-    if (pot)
-        hipLaunchKernelGGL(k_direct<true>, dim3(nb), dim3(64 * kDirectWaves), 0, st, pos, vel, ws->dblocks.p,
-                           ws->sims.p, nullptr, ws->pe.p);
-    else
-        hipLaunchKernelGGL(k_direct<false>, dim3(nb), dim3(64 * kDirectWaves), 0, st, pos, vel, ws->dblocks.p,
-                           ws->sims.p, force, nullptr);
+    const dim3 grid(nb), block(64 * kDirectWaves);
+    const DirectBlock* bl = ws->dblocks.p;
+    if (pot) {
+        if (p.direct_lanes == 4)
+            hipLaunchKernelGGL((k_direct<true, 4>), grid, block, 0, st, pos, vel, bl, ws->sims.p, nullptr, ws->pe.p);
+        else if (p.direct_lanes == 2)
+            hipLaunchKernelGGL((k_direct<true, 2>), grid, block, 0, st, pos, vel, bl, ws->sims.p, nullptr, ws->pe.p);
+        else
+            hipLaunchKernelGGL((k_direct<true, 1>), grid, block, 0, st, pos, vel, bl, ws->sims.p, nullptr, ws->pe.p);
+    } else {
+        if (p.direct_lanes == 4)
+            hipLaunchKernelGGL((k_direct<false, 4>), grid, block, 0, st, pos, vel, bl, ws->sims.p, force, nullptr);
+        else if (p.direct_lanes == 2)
+            hipLaunchKernelGGL((k_direct<false, 2>), grid, block, 0, st, pos, vel, bl, ws->sims.p, force, nullptr);
+        else
+            hipLaunchKernelGGL((k_direct<false, 1>), grid, block, 0, st, pos, vel, bl, ws->sims.p, force, nullptr);
+    }
     GI_HIP(hipGetLastError());
     if (timed) {
         GI_HIP(hipEventRecord(t0.b, st));

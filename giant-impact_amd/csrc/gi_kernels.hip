@@ -599,40 +599,46 @@ __global__ __launch_bounds__(256) void k_finish(float4* __restrict__ force, int 
 // ---------------------------------------------------------------------------
 //
 // For small N the three passes above are bound by their dozen launches, not by
-// arithmetic.  k_direct computes the same force in one: a workgroup holds 64
-// i-particles (one per lane) and its kDirectWaves wavefronts share the
-// simulation's j-range, each segment (iron j, then silicate j, so the gravity
-// product is one factor per segment) cut into groups of four j dealt out in
-// turn (wavefront w takes groups w, w + kDirectWaves, ...).  Dealing keeps
-// the contact work -- concentrated on the j near the block's own particles --
-// spread over all wavefronts.  Every pair takes the all-pairs pass's
-// branch-free gravity term (j through wave-uniform scalar loads; float tiles
-// of kDirectTile j added in FP64); when some lane of a group has a pair with
-// r^2 < 1 the group also adds, for its pairs in contact, the exact contact
-// term minus that gravity term, with the arithmetic of k_finish.  The
-// wavefronts' sums are added in FP64 in a fixed order, so the result depends
-// on the simulation alone.  POT: the per-particle potential of
-// k_finish_potential instead of the force.
+// arithmetic.  k_direct computes the same force in one: a workgroup holds
+// 64 / S i-particles, S lanes each, and its kDirectWaves * S lanes per
+// particle ("j-streams") share the simulation's j-range: each segment (iron
+// j, then silicate j, so the gravity product is one factor per segment) is
+// cut into groups of four j dealt out in turn (stream k takes groups k,
+// k + 16 S, ...).  Dealing keeps the contact work -- concentrated on the j
+// near the block's own particles -- spread over all streams; S > 1 gives small
+// simulations S times more wavefronts, each with a shorter chain.  With S = 1
+// the j come through wave-uniform scalar loads.  Every pair takes the
+// all-pairs pass's branch-free gravity term (float tiles of kDirectTile j
+// added in FP64); when some lane of a group has a pair with r^2 < 1 the group
+// also adds, for its pairs in contact, the exact contact term minus that
+// gravity term, with the arithmetic of k_finish.  The streams' sums are added
+// in FP64 in a fixed order, so the result depends on the simulation alone.
+// POT: the per-particle potential of k_finish_potential instead of the force.
 constexpr int kDirectWaves = 16;
 constexpr int kDirectTile = 256;  // j per float tile
 // r^2 (with FMAs and kSelfEps) of every pair whose exactly rounded r is below
 // 1 lies below this: the margin covers the few ulps between the two forms.
 constexpr float kDirectContactR2 = 1.0001f;
 
-template <bool POT>
+template <bool POT, int S>
 __global__ __launch_bounds__(64 * kDirectWaves) void k_direct(const float4* __restrict__ pos,
                                                               const float4* __restrict__ vel,
                                                               const DirectBlock* __restrict__ blocks,
                                                               const SimDesc* __restrict__ sims,
                                                               float4* __restrict__ force,
                                                               double* __restrict__ pe) {
+    static_assert(S == 1 || S == 2 || S == 4, "lanes per particle");
+    constexpr int IB = 64 / S;               // i-particles per workgroup
+    constexpr int NS = kDirectWaves * S;     // j-streams per particle
+    constexpr int TG = kDirectTile / 4;      // groups per stream per float tile
     __shared__ double red[3][kDirectWaves][64];
     const int lane = threadIdx.x & 63;
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int stream = wv * S + lane / IB;  // wave-uniform when S == 1
     const DirectBlock b = blocks[blockIdx.x];
     const int start = sims[b.sim].start, fe_end = sims[b.sim].fe_end, end = sims[b.sim].end;
     const PairLaw law = sims[b.sim].law;  // in registers: the contact branch reads it often
-    const int i = b.i0 + lane;
+    const int i = b.i0 + lane % IB;
     const bool live = i < end;
     const int ic = live ? i : end - 1;
     const float4 pi = pos[ic];
@@ -646,7 +652,7 @@ __global__ __launch_bounds__(64 * kDirectWaves) void k_direct(const float4* __re
         const int a = jfe ? start : fe_end;
         const int e = jfe ? fe_end : end;
         const int ngroups = (e - a + 3) / 4;
-        if (wv >= ngroups) continue;
+        if (stream >= ngroups) continue;
         const int test = (int)ife + (int)jfe;
         const float g = gravity_product(law, test);
         double ux = 0.0, uy = 0.0, uz = 0.0;  // float tiles of kDirectTile j, folded in FP64
@@ -683,9 +689,9 @@ __global__ __launch_bounds__(64 * kDirectWaves) void k_direct(const float4* __re
                 }
             }
         };
-        // this wavefront's groups, kDirectTile / 4 of them per float tile
-        for (int g0 = wv; g0 < ngroups; g0 += kDirectWaves * (kDirectTile / 4)) {
-            const int g1 = min(g0 + kDirectWaves * (kDirectTile / 4), ngroups);
+        // this stream's groups, TG of them per float tile
+        for (int g0 = stream; g0 < ngroups; g0 += NS * TG) {
+            const int g1 = min(g0 + NS * TG, ngroups);
             float tx = 0.0f, ty = 0.0f, tz = 0.0f;
             auto pair = [&](const float4 q, float& s, bool& near) {
                 const float dx = q.x - pi.x;
@@ -711,9 +717,9 @@ __global__ __launch_bounds__(64 * kDirectWaves) void k_direct(const float4* __re
             float4 q[4];
 #pragma unroll
             for (int u = 0; u < 4; ++u) q[u] = pos[min(a + 4 * g0 + u, e - 1)];
-            for (int grp = g0; grp < g1; grp += kDirectWaves) {
+            for (int grp = g0; grp < g1; grp += NS) {
                 const int j = a + 4 * grp;
-                const int jn = grp + kDirectWaves < g1 ? j + 4 * kDirectWaves : j;
+                const int jn = grp + NS < g1 ? j + 4 * NS : j;
                 float4 nq[4];
 #pragma unroll
                 for (int u = 0; u < 4; ++u) nq[u] = pos[min(jn + u, e - 1)];
@@ -755,14 +761,16 @@ __global__ __launch_bounds__(64 * kDirectWaves) void k_direct(const float4* __re
     red[1][wv][lane] = sy;
     red[2][wv][lane] = sz;
     __syncthreads();
-    if (wv == 0 && live) {
+    if (wv == 0 && lane < IB && live) {
         double fx = 0.0, fy = 0.0, fz = 0.0;
 #pragma unroll
-        for (int w = 0; w < kDirectWaves; ++w) {
-            fx += red[0][w][lane];
-            fy += red[1][w][lane];
-            fz += red[2][w][lane];
-        }
+        for (int w = 0; w < kDirectWaves; ++w)
+#pragma unroll
+            for (int q = 0; q < S; ++q) {
+                fx += red[0][w][q * IB + lane];
+                fy += red[1][w][q * IB + lane];
+                fz += red[2][w][q * IB + lane];
+            }
         if constexpr (!POT) {
             force[i].x = (float)fx;
             force[i].y = (float)fy;
@@ -1076,10 +1084,15 @@ GI_GRAVITY_INST(8, true, 0)
 #undef GI_GRAVITY_INST
 #define GI_MUTUAL_INST(R, POT) \
     template __global__ void k_mutual<R, POT>(const float4*, const MutualTask*, int, const SimDesc*, float*, float*);
-template __global__ void k_direct<false>(const float4*, const float4*, const DirectBlock*, const SimDesc*, float4*,
-                                        double*);
-template __global__ void k_direct<true>(const float4*, const float4*, const DirectBlock*, const SimDesc*, float4*,
-                                       double*);
+#define GI_DIRECT_INST(POT, S)                                                                               \
+    template __global__ void k_direct<POT, S>(const float4*, const float4*, const DirectBlock*, const SimDesc*, \
+                                              float4*, double*);
+GI_DIRECT_INST(false, 1)
+GI_DIRECT_INST(false, 2)
+GI_DIRECT_INST(false, 4)
+GI_DIRECT_INST(true, 1)
+GI_DIRECT_INST(true, 2)
+GI_DIRECT_INST(true, 4)
 GI_MUTUAL_INST(1, false)
 GI_MUTUAL_INST(2, false)
 GI_MUTUAL_INST(4, false)

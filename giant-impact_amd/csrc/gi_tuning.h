@@ -5,7 +5,8 @@
 //              rcp (1 multiply, 2 transcendental ops); 2 = mutual pass (each
 //              pair once for both particles, rows of 64*R, rsq cubed);
 //              3 = direct pass (small simulations: the whole force in one
-//              launch, 64 i-particles per workgroup; R and the sizes unused)
+//              launch; R = lanes per i-particle, 1, 2 or 4 (8 acts as 4),
+//              64 / R i-particles per workgroup; the sizes unused)
 // chunk_len    j-particles per task (0 = derive from target_tasks)
 // target_tasks wavefront tasks to aim for (several per SIMD of the 256 CUs x
 //              4 SIMDs so the tail of the launch stays short)

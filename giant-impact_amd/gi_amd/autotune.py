@@ -30,8 +30,11 @@ TUNING_DIR = Path(__file__).resolve().parent.parent / "tuning"
 
 # (n_max of the class, n_target, n_impactor of the representative impact)
 CLASSES = [
+    (3072, 1536, 512),
     (6144, 3072, 1024),
+    (12288, 6144, 2048),
     (24576, 12288, 4096),
+    (49152, 24576, 8192),
     (98304, 49152, 16384),
     (393216, 196608, 65536),
     (2097152, 786432, 262144),
@@ -40,7 +43,7 @@ CLASSES = [
 
 # classes up to this size also try the direct pass (one launch per force;
 # it wins where the three-pass form is bound by its launches)
-DIRECT_MAX_CLASS = 24576
+DIRECT_MAX_CLASS = 49152
 
 
 def time_gravity(n_target: int, n_impactor: int, reps: int):
@@ -139,7 +142,7 @@ def main(argv=None) -> int:
         cands = [(R, v, 0, tt, mc) for R, v, tt in itertools.product(Rs, variants, targets)]
         cands += [(R, 2, L, 0, 64) for R, L in itertools.product([4, 8], mutual_Ls)]
         if n_max <= DIRECT_MAX_CLASS:
-            cands.append((1, 3, 0, 0, 0))
+            cands += [(S, 3, 0, 0, 0) for S in (1, 2, 4)]  # R = lanes per i-particle
         best = None
         for cand in cands:
             call("gi_set_tiling", *cand)

@@ -295,7 +295,8 @@ int gi_find_earth_moon(const gi_float4 *pos, const gi_float4 *vel, int n, int nf
  * lane (1, 2, 4 or 8), variant = the all-pairs pass (0: one-sided, r^-3 as rsq
  * cubed; 1: one-sided, rsq*rcp; 2: mutual -- each pair evaluated once for both
  * particles, rows of 64*r particles; 3: direct -- small simulations, the whole
- * force, contacts included, in one launch), chunk_len = j-particles per task (0 =
+ * force, contacts included, in one launch, r = lanes per i-particle),
+ * chunk_len = j-particles per task (0 =
  * derived from target_tasks),
  * target_tasks = wavefront tasks to aim for, min_chunk = lower bound of
  * chunk_len.  r = 0 restores the table.  Plans built with another tiling are

@@ -1,5 +1,5 @@
-"""The direct pass (tiling variant 3): the whole force of a small simulation,
-contacts included, in one launch.  Checked against the oracle for the impact
+"""The direct pass (tiling variant 3, with 1, 2 or 4 lanes per i-particle):
+the whole force of a small simulation, contacts included, in one launch.  Checked against the oracle for the impact
 and generator forms, ragged and one-material sizes, a state with receding
 deep contacts, the potential energy, agreement with the three-pass forms,
 and the graph-captured stepper.  Tolerances: tests/parity.py.
@@ -17,10 +17,11 @@ from parity import assert_force_parity, force_metrics
 pytestmark = pytest.mark.gpu
 
 
-@pytest.fixture
-def direct(gpu):
-    gpu.call("gi_set_tiling", 1, 3, 0, 0, 0)
-    yield
+@pytest.fixture(params=[1, 2, 4], ids=lambda s: f"lanes{s}")
+def direct(gpu, request):
+    """The direct pass with 1, 2 or 4 lanes (j-streams) per i-particle."""
+    gpu.call("gi_set_tiling", request.param, 3, 0, 0, 0)
+    yield request.param
     gpu.call("gi_set_tiling", 0, 0, 0, 0, 0)
 
 
@@ -115,15 +116,15 @@ def test_direct_agrees_with_the_three_pass_forms(gpu):
     pos, vel, fc, _, _ = gpu.make_impact(6000, 2000, seed=12, angle_deg=40.0, speed_vesc=1.1, target_spin=0.3)
     print("n=8000", flush=True)
     Fs = {}
-    for name, tiling in (("direct", (1, 3, 0, 0, 0)), ("one-sided", (1, 0, 0, 8192, 512)),
-                         ("mutual", (8, 2, 512, 0, 64))):
+    for name, tiling in (("direct", (1, 3, 0, 0, 0)), ("direct4", (4, 3, 0, 0, 0)),
+                         ("one-sided", (1, 0, 0, 8192, 512)), ("mutual", (8, 2, 512, 0, 64))):
         gpu.call("gi_set_tiling", *tiling)
         try:
             Fs[name] = _force(gpu, pos, vel, fc)[:, :3].astype(np.float64)
         finally:
             gpu.call("gi_set_tiling", 0, 0, 0, 0, 0)
     ref = Fs["direct"]
-    for name in ("one-sided", "mutual"):
+    for name in ("direct4", "one-sided", "mutual"):
         d = np.linalg.norm(Fs[name] - ref, axis=1) / np.maximum(np.linalg.norm(ref, axis=1), 1e-30)
         print(name, d.max(), np.percentile(d, 99.9))
         assert d.max() <= 5e-6 and np.percentile(d, 99.9) <= 1e-6

@@ -192,6 +192,45 @@ def test_graph_stepper_equals_the_manual_loop(gpu):
     gpu.call("gi_stream_destroy", stream)
 
 
+@pytest.mark.parametrize("rate", [3, 8, 10])
+def test_graph_stepper_with_recenter_equals_the_manual_loop(gpu, rate):
+    """25 steps with a recenter every `rate` steps: the one- and eight-step
+    graphs and the recenter graph, against single calls, bitwise."""
+    pos, vel, fc, mc, _ = gpu.make_impact(1536, 512, seed=13, angle_deg=30.0, speed_vesc=1.3)
+    n = len(pos)
+    tc = (C.c_double * 3)(0.5, -1.0, 2.0)
+    tv = (C.c_double * 3)(0.01, 0.0, -0.02)
+    mfe = float(np.float64(np.float32(mc.Dt)) / np.float64(np.float32(mc.DtOverMassFe)))
+    msi = float(np.float64(np.float32(mc.Dt)) / np.float64(np.float32(mc.DtOverMassSi)))
+    print(f"rate={rate}", flush=True)
+    P1, V1, F1 = _upload(gpu, pos, vel)
+    for s in range(1, 26):
+        gpu.force_collision(P1, V1, F1, n, fc)
+        gpu.move_collision(P1, V1, F1, n, mc)
+        if s % rate == 0:
+            gpu.call("gi_recenter", P1.ptr, V1.ptr, n, fc.NFe, mfe, msi, tc, tv, None)
+    P2, V2, F2 = _upload(gpu, pos, vel)
+    gpu.call("gi_impact_steps", P2.ptr, V2.ptr, F2.ptr, n, C.byref(fc), C.byref(mc), 25, rate, tc, tv, None)
+    gpu.sync()
+    assert np.array_equal(P1.download(), P2.download())
+    assert np.array_equal(V1.download(), V2.download())
+
+
+def test_separate_stepper_equals_the_manual_loop(gpu):
+    pos, vel, fs, ms = _generator_bodies(gpu, 1200, 500, 31)
+    n = len(pos)
+    P1, V1, F1 = _upload(gpu, pos, vel)
+    for _ in range(19):
+        gpu.force_separate(P1, V1, F1, n, fs)
+        gpu.move_damped_separate(P1, V1, F1, n, ms, 6.0, 2.0)
+    P2, V2, F2 = _upload(gpu, pos, vel)
+    gpu.call("gi_separate_steps", P2.ptr, V2.ptr, F2.ptr, n, C.byref(fs), C.byref(ms), C.c_float(6.0),
+             C.c_float(2.0), 19, None)
+    gpu.sync()
+    assert np.array_equal(P1.download(), P2.download())
+    assert np.array_equal(V1.download(), V2.download())
+
+
 def test_steps_track_the_reference_for_a_few_steps(gpu, oracle):
     pos, vel, fc, mc, _ = gpu.make_impact(1536, 512, seed=20261202, angle_deg=45, speed_vesc=1.2)
     n = len(pos)
