@@ -555,20 +555,28 @@ __global__ __launch_bounds__(256) void k_finish(float4* __restrict__ force, int 
     // cancellation between large contact forces adds no summation error.
     // Cells are visited in a fixed order and a bucket lists its particles by
     // index, so the order of the terms does not depend on the rest of the
-    // launch.
+    // launch.  A simulation's buckets hold only its own particles, and i
+    // itself sits at position t, so a candidate's index is loaded only for a
+    // pair in contact; the three bucket ranges of a row of cells are loaded
+    // together.
     double ex = 0.0, ey = 0.0, ez = 0.0;
     for (int oz = -1; oz <= 1; ++oz)
-        for (int oy = -1; oy <= 1; ++oy)
-            for (int ox = -1; ox <= 1; ++ox) {
-                const int qx = ccx + ox, qy = ccy + oy, qz = ccz + oz;
-                const unsigned h = cell_bucket(sd.hbase, sd.hmask, qx, qy, qz);
-                const int b0 = starts[h], b1 = starts[h + 1];
-                for (int k = b0; k < b1; ++k) {
+        for (int oy = -1; oy <= 1; ++oy) {
+            const int qy = ccy + oy, qz = ccz + oz;
+            int b0[3], b1[3];
+#pragma unroll
+            for (int ox = 0; ox < 3; ++ox) {
+                const unsigned h = cell_bucket(sd.hbase, sd.hmask, ccx + ox - 1, qy, qz);
+                b0[ox] = starts[h];
+                b1[ox] = starts[h + 1];
+            }
+#pragma unroll
+            for (int ox = 0; ox < 3; ++ox) {
+                const int qx = ccx + ox - 1;
+                for (int k = b0[ox]; k < b1[ox]; ++k) {
                     const float4 pj = spos[k];
-                    if (cell_coord(pj.x) != qx || cell_coord(pj.y) != qy || cell_coord(pj.z) != qz)
-                        continue;  // another cell sharing the bucket
-                    const int j = sorted[k];
-                    if (j == i || j < sd.start || j >= sd.end) continue;
+                    if (k == t || cell_coord(pj.x) != qx || cell_coord(pj.y) != qy || cell_coord(pj.z) != qz)
+                        continue;  // i itself, or another cell sharing the bucket
                     const float dpx = pj.x - pi.x;
                     const float dpy = pj.y - pi.y;
                     const float dpz = pj.z - pi.z;
@@ -576,6 +584,7 @@ __global__ __launch_bounds__(256) void k_finish(float4* __restrict__ force, int 
                     const float r = sqrtf(r2);
                     if (1.0f <= r) continue;
                     const float invr = 1.0f / r;
+                    const int j = sorted[k];
                     const int test = (int)ife + (int)(j < sd.fe_end);
                     const float4 vj = svel[k];
                     const float dvx = vj.x - vi.x, dvy = vj.y - vi.y, dvz = vj.z - vi.z;
@@ -589,6 +598,7 @@ __global__ __launch_bounds__(256) void k_finish(float4* __restrict__ force, int 
                     ez += (double)(fm * dpz * invr) - (double)(sf * dpz);
                 }
             }
+        }
     force[i].x = (float)((double)fx + ex);
     force[i].y = (float)((double)fy + ey);
     force[i].z = (float)((double)fz + ez);
