@@ -165,6 +165,10 @@ struct DevCtx {
     DevBuf<double> rc_blocks, rc_sums;
     hipStream_t capture_stream = nullptr;
     hipEvent_t ev_in = nullptr, ev_out = nullptr;
+    // side stream of a force evaluation (default priority): the contact grid
+    // is built on it while the all-pairs pass runs
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     std::vector<GraphEntry> graphs;
     std::vector<TimingPair> pending, pool;
     long long timed_launches[kTimedKinds] = {0, 0, 0, 0};
@@ -685,8 +689,18 @@ static int run_force(DevCtx* ctx, Workspace* ws, const float4* pos, const float4
     if (p.direct) {
         if ((rc = launch_direct(ctx, ws, pos, vel, force, false, st, timed))) return rc;
     } else {
-        if ((rc = launch_grid(ws, pos, vel, st))) return rc;
+        // The contact grid reads only pos / vel: it is built on the side
+        // stream while the all-pairs pass runs on st, and joined before
+        // k_finish (under graph capture the fork and join become edges).
+        if (!ctx->side) GI_HIP(hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking));
+        if (!ctx->ev_fork) GI_HIP(hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));
+        if (!ctx->ev_join) GI_HIP(hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming));
+        GI_HIP(hipEventRecord(ctx->ev_fork, st));
+        GI_HIP(hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
+        if ((rc = launch_grid(ws, pos, vel, ctx->side))) return rc;
+        GI_HIP(hipEventRecord(ctx->ev_join, ctx->side));
         if ((rc = launch_gravity(ctx, ws, pos, false, st, timed))) return rc;
+        GI_HIP(hipStreamWaitEvent(st, ctx->ev_join, 0));
         hipLaunchKernelGGL(k_finish, dim3(nblk(p.ntot, 256)), dim3(256), 0, st, force, p.ntot, ws->sims.p,
                            (int)p.sims.size(), ws->partial.p, p.mutual ? ws->gsum.p : nullptr, ws->starts.p,
                            ws->sorted.p, ws->spos.p, ws->svel.p, p.tiling.variant);
