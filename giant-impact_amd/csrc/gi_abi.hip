@@ -622,7 +622,7 @@ static int launch_gravity(DevCtx* ctx, Workspace* ws, const float4* pos, bool po
         ctx->pending.push_back(tp);
     }
     if (nmut > 0) {
-        hipLaunchKernelGGL(k_mutual_sum, dim3(nblk(p.ntot, 256)), dim3(256), 0, st, p.ntot, ws->sims.p, nsims,
+        hipLaunchKernelGGL(k_mutual_sum, dim3(nblk(p.ntot, 64)), dim3(64 * kSumSplit), 0, st, p.ntot, ws->sims.p, nsims,
                            ws->rows.p, 64 * p.tiling.R, ws->iside.p, ws->jside.p, ws->gsum.p, pot ? 1 : 0);
         GI_HIP(hipGetLastError());
     }

@@ -347,38 +347,64 @@ __global__ void k_mutual_pad(const float4* __restrict__ pos, int gtot, const Sim
 
 // Per particle (index order): the i-side slots of its row and the j-side
 // entries of every earlier row, summed in FP64 in a fixed order.
-__global__ __launch_bounds__(256) void k_mutual_sum(int ntot, const SimDesc* __restrict__ sims, int nsims,
-                                                    const RowDesc* __restrict__ rows, int B,
-                                                    const float* __restrict__ iside,
-                                                    const float* __restrict__ jside,
-                                                    float4* __restrict__ gsum, int pot) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= ntot) return;
-    const SimDesc& sd = sims[find_sim(sims, nsims, i)];
-    if (!sd.active || sd.gn == 0) return;  // gn == 0: one-sided pass
-    const int q = i < sd.fe_end ? i - sd.start : sd.gfe + (i - sd.fe_end);
-    const int row = q / B;
-    const int o = q - row * B;
-    const RowDesc rd = rows[sd.row0 + row];
+// Four wavefronts per 64 particles: wavefront w adds the w-th quarter of
+// each particle's slots and slab entries (one particle per lane, so the loads
+// stay coalesced), and the quarters are added in LDS in a fixed order --
+// four times the loads in flight of one thread per particle.
+constexpr int kSumSplit = 4;
+
+__global__ __launch_bounds__(64 * kSumSplit) void k_mutual_sum(int ntot, const SimDesc* __restrict__ sims,
+                                                              int nsims, const RowDesc* __restrict__ rows, int B,
+                                                              const float* __restrict__ iside,
+                                                              const float* __restrict__ jside,
+                                                              float4* __restrict__ gsum, int pot) {
+    __shared__ double red[3][kSumSplit][64];
+    const int lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+    const int i = blockIdx.x * 64 + lane;
     double sx = 0.0, sy = 0.0, sz = 0.0;
-    for (int k = 0; k < rd.nslot; ++k) {
-        const float* p = iside + (size_t)(rd.slot0 + k) * (3 * B) + o;
-        sx += (double)p[0];
-        if (!pot) {
-            sy += (double)p[B];
-            sz += (double)p[2 * B];
+    bool live = false;
+    if (i < ntot) {
+        const SimDesc& sd = sims[find_sim(sims, nsims, i)];
+        live = sd.active && sd.gn > 0;  // gn == 0: one-sided pass
+        if (live) {
+            const int q = i < sd.fe_end ? i - sd.start : sd.gfe + (i - sd.fe_end);
+            const int row = q / B;
+            const int o = q - row * B;
+            const RowDesc rd = rows[sd.row0 + row];
+            for (int k = rd.nslot * w / kSumSplit; k < rd.nslot * (w + 1) / kSumSplit; ++k) {
+                const float* p = iside + (size_t)(rd.slot0 + k) * (3 * B) + o;
+                sx += (double)p[0];
+                if (!pot) {
+                    sy += (double)p[B];
+                    sz += (double)p[2 * B];
+                }
+            }
+            for (int r = row * w / kSumSplit; r < row * (w + 1) / kSumSplit; ++r) {
+                const long long len = sd.gn - (long long)(r + 1) * B;
+                const float* p = jside + sd.jbase + 3 * jside_row_offset(r, sd.gn, B) + (q - (r + 1) * B);
+                sx += (double)p[0];
+                if (!pot) {
+                    sy += (double)p[len];
+                    sz += (double)p[2 * len];
+                }
+            }
         }
     }
-    for (int r = 0; r < row; ++r) {
-        const long long len = sd.gn - (long long)(r + 1) * B;
-        const float* p = jside + sd.jbase + 3 * jside_row_offset(r, sd.gn, B) + (q - (r + 1) * B);
-        sx += (double)p[0];
-        if (!pot) {
-            sy += (double)p[len];
-            sz += (double)p[2 * len];
+    red[0][w][lane] = sx;
+    red[1][w][lane] = sy;
+    red[2][w][lane] = sz;
+    __syncthreads();
+    if (w == 0 && live) {
+        double tx = 0.0, ty = 0.0, tz = 0.0;
+#pragma unroll
+        for (int k = 0; k < kSumSplit; ++k) {
+            tx += red[0][k][lane];
+            ty += red[1][k][lane];
+            tz += red[2][k][lane];
         }
+        gsum[i] = make_float4((float)tx, (float)ty, (float)tz, 0.0f);
     }
-    gsum[i] = make_float4((float)sx, (float)sy, (float)sz, 0.0f);
 }
 
 // ---------------------------------------------------------------------------
