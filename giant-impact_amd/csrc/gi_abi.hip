@@ -358,7 +358,8 @@ static int build_plan(std::vector<SimDesc> sims, const Tiling& tl, bool batched,
         const int ib = 64 / p.direct_lanes;
         for (size_t si = 0; si < sims.size(); ++si)
             if (sims[si].active)
-                for (int i0 = sims[si].start; i0 < sims[si].end; i0 += ib) p.dblocks.push_back(DirectBlock{(int)si, i0});
+                for (int i0 = sims[si].start; i0 < sims[si].end; i0 += ib)
+                    p.dblocks.push_back(DirectBlock{(int)si, i0});
         p.direct = true;
     }
     long long pbase = 0;
@@ -622,8 +623,8 @@ static int launch_gravity(DevCtx* ctx, Workspace* ws, const float4* pos, bool po
         ctx->pending.push_back(tp);
     }
     if (nmut > 0) {
-        hipLaunchKernelGGL(k_mutual_sum, dim3(nblk(p.ntot, 64)), dim3(64 * kSumSplit), 0, st, p.ntot, ws->sims.p, nsims,
-                           ws->rows.p, 64 * p.tiling.R, ws->iside.p, ws->jside.p, ws->gsum.p, pot ? 1 : 0);
+        hipLaunchKernelGGL(k_mutual_sum, dim3(nblk(p.ntot, 64)), dim3(64 * kSumSplit), 0, st, p.ntot, ws->sims.p,
+                           nsims, ws->rows.p, 64 * p.tiling.R, ws->iside.p, ws->jside.p, ws->gsum.p, pot ? 1 : 0);
         GI_HIP(hipGetLastError());
     }
     if (timed) {
@@ -1020,7 +1021,9 @@ int gi_plan_info(int* mutual, int* r, long long* pair_tasks, long long* chunk_le
     if (mutual) *mutual = !valid ? 0 : (p.direct ? 2 : (p.mutual ? 1 : 0));
     if (r) *r = valid ? p.tiling.R : 0;
     if (pair_tasks)
-        *pair_tasks = !valid ? 0 : (p.direct ? (long long)p.dblocks.size() : (long long)(p.tasks.size() + p.mtasks.size()));
+        *pair_tasks = !valid     ? 0
+                      : p.direct ? (long long)p.dblocks.size()
+                                 : (long long)(p.tasks.size() + p.mtasks.size());
     if (chunk_len) *chunk_len = valid ? p.chunk_len : 0;
     return 0;
 }
@@ -1229,7 +1232,8 @@ int gi_find_earth_moon(const gi_float4* pos, const gi_float4* vel, int n, int nf
     const int moon = best[1] ? (int)~(unsigned)(best[1] & 0xffffffffull) : -1;
     // moments of the system, the Earth and the Moon
     const int nb = std::min(1024, nblk(n, kCCBlock));
-    if ((rc = w.moments.ensure((size_t)nb * kCCClasses * kCCMoments)) || (rc = w.mcounts.ensure((size_t)nb * kCCCounts)))
+    if ((rc = w.moments.ensure((size_t)nb * kCCClasses * kCCMoments)) ||
+        (rc = w.mcounts.ensure((size_t)nb * kCCCounts)))
         return rc;
     hipLaunchKernelGGL(k_cc_moments, dim3(nb), dim3(kCCBlock), 0, st, P, (const float4*)vel, n, w.parent.p, earth,
                        moon, nfe1, nfe2, nsi1, mass_fe, mass_si, w.moments.p, w.mcounts.p);
@@ -1246,7 +1250,8 @@ int gi_find_earth_moon(const gi_float4* pos, const gi_float4* vel, int n, int nf
     double M[4][kCCMoments] = {};
     for (int b = 0; b < nb; ++b)
         for (int c = 0; c < kCCClasses; ++c)
-            for (int k = 0; k < kCCMoments; ++k) M[c == 0 ? 0 : c + 1][k] += mom[((size_t)b * kCCClasses + c) * kCCMoments + k];
+            for (int k = 0; k < kCCMoments; ++k)
+                M[c == 0 ? 0 : c + 1][k] += mom[((size_t)b * kCCClasses + c) * kCCMoments + k];
     for (int k = 0; k < kCCMoments; ++k) M[1][k] = M[2][k] + M[3][k];
     std::memset(out, 0, sizeof *out);
     int c8[kCCCounts] = {};

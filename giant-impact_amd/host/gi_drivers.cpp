@@ -153,13 +153,15 @@ int gi_write_run_stats(const char *path, const gi_earth_moon *em, const gi_units
     {
         const double x = em->vel[0][0] * velocity_c, y = em->vel[0][1] * velocity_c, z = em->vel[0][2] * velocity_c;
         std::fprintf(f, "\nLinear velocity of the entire system system \t\t= (%f, %f, %f)", x, y, z);
-        std::fprintf(f, "\nMagnitude of the linear velocity of the entire system \t= %f\n", std::sqrt(x * x + y * y + z * z));
+        std::fprintf(f, "\nMagnitude of the linear velocity of the entire system \t= %f\n",
+                     std::sqrt(x * x + y * y + z * z));
     }
     {
         const double x = em->angmom[0][0] * momentum_c, y = em->angmom[0][1] * momentum_c,
                      z = em->angmom[0][2] * momentum_c;
         std::fprintf(f, "\nAngular momentum of the entire system system \t\t= (%e, %e, %e)", x, y, z);
-        std::fprintf(f, "\nMagnitude of the angular momentum of the entire system \t= %e\n", std::sqrt(x * x + y * y + z * z));
+        std::fprintf(f, "\nMagnitude of the angular momentum of the entire system \t= %e\n",
+                     std::sqrt(x * x + y * y + z * z));
     }
     std::fprintf(f, "\n\n*************************************************************************\n\n\n");
     return std::fclose(f) == 0 ? 0 : GI_E_IO;

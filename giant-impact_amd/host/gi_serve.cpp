@@ -138,7 +138,8 @@ extern "C" int gi_serve_run(const gi_serve_request *reqs, int n_requests, const 
             char *dv = (char *)buf[dst].vel + (size_t)off * 16;
             int r;
             if (j.fresh) {
-                if ((r = gi_copy_h2d(dp, j.pos0.data(), bytes)) || (r = gi_copy_h2d(dv, j.vel0.data(), bytes))) return r;
+                if ((r = gi_copy_h2d(dp, j.pos0.data(), bytes)) || (r = gi_copy_h2d(dv, j.vel0.data(), bytes)))
+                    return r;
             } else {
                 const char *sp = (const char *)buf[cur].pos + (size_t)j.offset * 16;
                 const char *sv = (const char *)buf[cur].vel + (size_t)j.offset * 16;
