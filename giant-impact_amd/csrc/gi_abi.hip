@@ -169,6 +169,10 @@ struct DevCtx {
     // is built on it while the all-pairs pass runs
     hipStream_t side = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    // second state of the steppers' multi-step graphs with the move fused
+    // into the direct pass (steps alternate between the caller's arrays and
+    // these)
+    DevBuf<float4> alt_pos, alt_vel;
     std::vector<GraphEntry> graphs;
     std::vector<TimingPair> pending, pool;
     long long timed_launches[kTimedKinds] = {0, 0, 0, 0};
@@ -634,10 +638,23 @@ static int launch_gravity(DevCtx* ctx, Workspace* ws, const float4* pos, bool po
     return 0;
 }
 
-// The direct pass of the current plan (force, or with pot the per-particle
-// potential into ws->pe).  Timed as both kind 0 and kind 2.
+template <bool POT, int MOVE>
+static void direct_kernel(int lanes, dim3 grid, dim3 block, hipStream_t st, const float4* pos, const float4* vel,
+                          const Workspace* ws, float4* force, double* pe, const FusedMove& mv) {
+    const DirectBlock* bl = ws->dblocks.p;
+    if (lanes == 4)
+        hipLaunchKernelGGL((k_direct<POT, 4, MOVE>), grid, block, 0, st, pos, vel, bl, ws->sims.p, force, pe, mv);
+    else if (lanes == 2)
+        hipLaunchKernelGGL((k_direct<POT, 2, MOVE>), grid, block, 0, st, pos, vel, bl, ws->sims.p, force, pe, mv);
+    else
+        hipLaunchKernelGGL((k_direct<POT, 1, MOVE>), grid, block, 0, st, pos, vel, bl, ws->sims.p, force, pe, mv);
+}
+
+// The direct pass of the current plan: the force, or with pot the
+// per-particle potential into ws->pe, or with move = 1 / 2 the force and the
+// collision / damped move into mv's arrays.  Timed as both kind 0 and kind 2.
 static int launch_direct(DevCtx* ctx, Workspace* ws, const float4* pos, const float4* vel, float4* force, bool pot,
-                         hipStream_t st, bool timed) {
+                         hipStream_t st, bool timed, int move = 0, const FusedMove& mv = FusedMove{}) {
     const Plan& p = ws->plan;
     const int nb = (int)p.dblocks.size();
     if (nb == 0) return 0;
@@ -649,22 +666,15 @@ static int launch_direct(DevCtx* ctx, Workspace* ws, const float4* pos, const fl
         GI_HIP(hipEventRecord(t0.a, st));
     }
     const dim3 grid(nb), block(64 * kDirectWaves);
-    const DirectBlock* bl = ws->dblocks.p;
-    if (pot) {
-        if (p.direct_lanes == 4)
-            hipLaunchKernelGGL((k_direct<true, 4>), grid, block, 0, st, pos, vel, bl, ws->sims.p, nullptr, ws->pe.p);
-        else if (p.direct_lanes == 2)
-            hipLaunchKernelGGL((k_direct<true, 2>), grid, block, 0, st, pos, vel, bl, ws->sims.p, nullptr, ws->pe.p);
-        else
-            hipLaunchKernelGGL((k_direct<true, 1>), grid, block, 0, st, pos, vel, bl, ws->sims.p, nullptr, ws->pe.p);
-    } else {
-        if (p.direct_lanes == 4)
-            hipLaunchKernelGGL((k_direct<false, 4>), grid, block, 0, st, pos, vel, bl, ws->sims.p, force, nullptr);
-        else if (p.direct_lanes == 2)
-            hipLaunchKernelGGL((k_direct<false, 2>), grid, block, 0, st, pos, vel, bl, ws->sims.p, force, nullptr);
-        else
-            hipLaunchKernelGGL((k_direct<false, 1>), grid, block, 0, st, pos, vel, bl, ws->sims.p, force, nullptr);
-    }
+    const int lanes = p.direct_lanes;
+    if (pot)
+        direct_kernel<true, 0>(lanes, grid, block, st, pos, vel, ws, nullptr, ws->pe.p, mv);
+    else if (move == 1)
+        direct_kernel<false, 1>(lanes, grid, block, st, pos, vel, ws, force, nullptr, mv);
+    else if (move == 2)
+        direct_kernel<false, 2>(lanes, grid, block, st, pos, vel, ws, force, nullptr, mv);
+    else
+        direct_kernel<false, 0>(lanes, grid, block, st, pos, vel, ws, force, nullptr, mv);
     GI_HIP(hipGetLastError());
     if (timed) {
         GI_HIP(hipEventRecord(t0.b, st));
@@ -850,6 +860,23 @@ static int get_graph(DevCtx* ctx, std::vector<GraphEntry>& cache, const std::vec
 // Steps per graph of the multi-step form: a whole graph launch per step is
 // a fixed cost that small simulations feel.
 constexpr int kStepsPerGraph = 8;
+
+// kStepsPerGraph steps of a direct-pass plan with the move fused into the
+// pass: step k reads one state and writes the other (the caller's arrays P, V
+// and ctx->alt_*), so after an even number of steps the state is back in P, V.
+static_assert(kStepsPerGraph % 2 == 0, "fused steps must end in the caller's arrays");
+static int fused_direct_steps(DevCtx* ctx, float4* P, float4* V, float4* F, FusedMove mv, int move, hipStream_t cs) {
+    float4 *sp = P, *sv = V, *dp = ctx->alt_pos.p, *dv = ctx->alt_vel.p;
+    for (int k = 0; k < kStepsPerGraph; ++k) {
+        mv.pout = dp;
+        mv.vout = dv;
+        int r = launch_direct(ctx, &ctx->ws, sp, sv, F, false, cs, false, move, mv);
+        if (r) return r;
+        std::swap(sp, dp);
+        std::swap(sv, dv);
+    }
+    return 0;
+}
 
 // Launch `count` steps: the kStepsPerGraph-step graph as often as it fits,
 // then the one-step graph.
@@ -1354,8 +1381,19 @@ int gi_impact_steps(gi_float4* pos, gi_float4* vel, gi_float4* force, int n, con
     if (rc) return rc;
     if (n_steps >= kStepsPerGraph && (recenter_rate == 0 || recenter_rate >= kStepsPerGraph)) {
         key_append(key, kStepsPerGraph);
-        rc = get_graph(ctx, ctx->graphs, key, ctx->ws.generation,
-                       [&](hipStream_t cs) { return body(cs, kStepsPerGraph); }, &steps, {step});
+        if (ctx->ws.plan.direct) {
+            // the move fused into the direct pass, alternating states
+            if ((rc = ctx->alt_pos.ensure((size_t)n)) || (rc = ctx->alt_vel.ensure((size_t)n))) return rc;
+            key_append(key, ctx->alt_pos.p);
+            key_append(key, ctx->alt_vel.p);
+            const FusedMove mv{nullptr, nullptr, m.Dt, m.DtOverMassFe, m.DtOverMassSi, m.NFe, 0, 0, 0.0f, 0.0f};
+            rc = get_graph(ctx, ctx->graphs, key, ctx->ws.generation,
+                           [&](hipStream_t cs) { return fused_direct_steps(ctx, P, V, F, mv, 1, cs); }, &steps,
+                           {step});
+        } else {
+            rc = get_graph(ctx, ctx->graphs, key, ctx->ws.generation,
+                           [&](hipStream_t cs) { return body(cs, kStepsPerGraph); }, &steps, {step});
+        }
         if (rc) return rc;
     }
     hipGraphExec_t rec = nullptr;
@@ -1442,8 +1480,20 @@ int gi_separate_steps(gi_float4* pos, gi_float4* vel, gi_float4* force, int n, c
     if (rc) return rc;
     if (n_steps >= kStepsPerGraph) {
         key_append(key, kStepsPerGraph);
-        rc = get_graph(ctx, ctx->graphs, key, ctx->ws.generation,
-                       [&](hipStream_t cs) { return body(cs, kStepsPerGraph); }, &steps, {step});
+        if (ctx->ws.plan.direct) {
+            // the damped move fused into the direct pass, alternating states
+            if ((rc = ctx->alt_pos.ensure((size_t)n)) || (rc = ctx->alt_vel.ensure((size_t)n))) return rc;
+            key_append(key, ctx->alt_pos.p);
+            key_append(key, ctx->alt_vel.p);
+            const FusedMove mv{nullptr,    nullptr,    m.Dt,       m.DtOverMassFe,  m.DtOverMassSi,
+                               m.boarder1, m.boarder2, m.boarder3, damp_rate_body1, damp_rate_body2};
+            rc = get_graph(ctx, ctx->graphs, key, ctx->ws.generation,
+                           [&](hipStream_t cs) { return fused_direct_steps(ctx, P, V, F, mv, 2, cs); }, &steps,
+                           {step});
+        } else {
+            rc = get_graph(ctx, ctx->graphs, key, ctx->ws.generation,
+                           [&](hipStream_t cs) { return body(cs, kStepsPerGraph); }, &steps, {step});
+        }
         if (rc) return rc;
     }
     if ((rc = launch_step_run(step, steps, n_steps, br.run))) return rc;

@@ -656,13 +656,28 @@ constexpr int kDirectTile = 256;  // j per float tile
 // 1 lies below this: the margin covers the few ulps between the two forms.
 constexpr float kDirectContactR2 = 1.0001f;
 
-template <bool POT, int S>
+// The move of a step fused into the direct pass (MOVE = 1:
+// moveBodiesCollisionSingleGPU, impactAddTime.cu:379-394; MOVE = 2:
+// moveBodiesDampedSeperate, generateTargetImpactor.cu:628-664), with the
+// same arithmetic as k_move_collision / k_move_damped_separate.  The new
+// state goes to other arrays (pout, vout): the pass still reads the old
+// positions of every particle.  MOVE = 1 reads b1 as NFe.
+struct FusedMove {
+    float4* pout;
+    float4* vout;
+    float dt, dt_over_mass_fe, dt_over_mass_si;
+    int b1, b2, b3;
+    float damp1, damp2;
+};
+
+template <bool POT, int S, int MOVE>
 __global__ __launch_bounds__(64 * kDirectWaves) void k_direct(const float4* __restrict__ pos,
                                                               const float4* __restrict__ vel,
                                                               const DirectBlock* __restrict__ blocks,
                                                               const SimDesc* __restrict__ sims,
                                                               float4* __restrict__ force,
-                                                              double* __restrict__ pe) {
+                                                              double* __restrict__ pe, FusedMove mv) {
+    static_assert(MOVE == 0 || !POT, "the potential pass moves nothing");
     static_assert(S == 1 || S == 2 || S == 4, "lanes per particle");
     constexpr int IB = 64 / S;               // i-particles per workgroup
     constexpr int NS = kDirectWaves * S;     // j-streams per particle
@@ -808,9 +823,44 @@ __global__ __launch_bounds__(64 * kDirectWaves) void k_direct(const float4* __re
                 fz += red[2][w][q * IB + lane];
             }
         if constexpr (!POT) {
-            force[i].x = (float)fx;
-            force[i].y = (float)fy;
-            force[i].z = (float)fz;
+            const float f0 = (float)fx, f1 = (float)fy, f2 = (float)fz;
+            force[i].x = f0;
+            force[i].y = f1;
+            force[i].z = f2;
+            if constexpr (MOVE != 0) {
+                float temp, damp = 0.0f;
+                if constexpr (MOVE == 1) {
+                    temp = i < mv.b1 ? mv.dt_over_mass_fe : mv.dt_over_mass_si;
+                } else if (mv.b3 <= i) {
+                    temp = mv.dt_over_mass_si;
+                    damp = mv.damp2;
+                } else if (mv.b2 <= i) {
+                    temp = mv.dt_over_mass_fe;
+                    damp = mv.damp2;
+                } else if (mv.b1 <= i) {
+                    temp = mv.dt_over_mass_si;
+                    damp = mv.damp1;
+                } else {
+                    temp = mv.dt_over_mass_fe;
+                    damp = mv.damp1;
+                }
+                float4 v = vel[i];
+                float4 p = pos[i];
+                if constexpr (MOVE == 1) {
+                    v.x += f0 * temp;
+                    v.y += f1 * temp;
+                    v.z += f2 * temp;
+                } else {
+                    v.x += (f0 - damp * v.x) * temp;
+                    v.y += (f1 - damp * v.y) * temp;
+                    v.z += (f2 - damp * v.z) * temp;
+                }
+                p.x += v.x * mv.dt;
+                p.y += v.y * mv.dt;
+                p.z += v.z * mv.dt;
+                mv.vout[i] = v;
+                mv.pout[i] = p;
+            }
         } else {
             pe[i] = 0.5 * fx;
         }
@@ -1120,15 +1170,17 @@ GI_GRAVITY_INST(8, true, 0)
 #undef GI_GRAVITY_INST
 #define GI_MUTUAL_INST(R, POT) \
     template __global__ void k_mutual<R, POT>(const float4*, const MutualTask*, int, const SimDesc*, float*, float*);
-#define GI_DIRECT_INST(POT, S)                                                                               \
-    template __global__ void k_direct<POT, S>(const float4*, const float4*, const DirectBlock*, const SimDesc*, \
-                                              float4*, double*);
-GI_DIRECT_INST(false, 1)
-GI_DIRECT_INST(false, 2)
-GI_DIRECT_INST(false, 4)
-GI_DIRECT_INST(true, 1)
-GI_DIRECT_INST(true, 2)
-GI_DIRECT_INST(true, 4)
+#define GI_DIRECT_INST(POT, S, MOVE)                                                                 \
+    template __global__ void k_direct<POT, S, MOVE>(const float4*, const float4*, const DirectBlock*, \
+                                                    const SimDesc*, float4*, double*, FusedMove);
+#define GI_DIRECT_INST_S(S)       \
+    GI_DIRECT_INST(false, S, 0)   \
+    GI_DIRECT_INST(false, S, 1)   \
+    GI_DIRECT_INST(false, S, 2)   \
+    GI_DIRECT_INST(true, S, 0)
+GI_DIRECT_INST_S(1)
+GI_DIRECT_INST_S(2)
+GI_DIRECT_INST_S(4)
 GI_MUTUAL_INST(1, false)
 GI_MUTUAL_INST(2, false)
 GI_MUTUAL_INST(4, false)
