@@ -77,6 +77,7 @@ def cpu_baseline(pos, vel, fc, n, target_s: float):
         "unit": "pair-int/s",
         "cores": threads,
         "kind": "port",
+        "ms_per_step_extrapolated": float(n) * n / (rows * n / dt) * 1e3,
         "sample": f"rows [0,{rows}) x all {n} j of one force evaluation of the N={n} impact "
         f"({rows * n:.3e} pairs, {dt:.1f} s) with the float restatement of getForcesCollisionSingleGPU, "
         f"gcc -O2 -ffp-contract=off, OpenMP over rows",
@@ -87,10 +88,13 @@ def ensemble_leg(gi, n_sims: int, steps: int, seq_steps: int = 100, check_every:
     """64 independent impacts of uneven N (U[4096, 32768], impactor fraction
     [0.1, 0.3], angle [30, 60] deg, speed [1.0, 1.4] v_esc, target spin [0, 0.6]
     breakup), batched (steps steps, FP64 E / L of every simulation every
-    check_every steps outside the timed region) vs each simulation stepped
-    alone with the single-simulation path (gi_impact_steps, also
-    graph-captured; seq_steps steps each, its per-simulation set-up excluded
-    from the sequential time); rates are compared per sim-step."""
+    check_every steps outside the timed region) against the same simulations
+    run one after another, seq_steps steps each, two ways: with the plain
+    drop-in kernels (gi_force_collision + gi_move_collision per step, as the
+    reference's step loop calls its two kernels) and with this library's
+    graph-captured single-simulation stepper (gi_impact_steps, the stronger
+    baseline); each simulation's set-up is excluded from the sequential
+    times, and rates are compared per sim-step."""
     cfg = gi.ensemble_configs(n_sims)
     pos, vel, n, fc, mc, _ = gi.make_ensemble(cfg)
     ntot = int(n.sum())
@@ -133,10 +137,25 @@ def ensemble_leg(gi, n_sims: int, steps: int, seq_steps: int = 100, check_every:
         gi.impact_steps(ptr(P), ptr(V), ptr(F), int(n[s]), fc[s], mc[s], seq_steps)
         gi.sync()
         t_seq += time.perf_counter() - t0
+    # sequential with the plain kernels: one force and one move call per step
+    P.upload(pos)
+    V.upload(vel)
+    t_plain = 0.0
+    for s in range(n_sims):
+        Ps, Vs, Fs = (A.value + 16 * int(off[s]) for A in (P, V, F))
+        gi.force_collision(Ps, Vs, Fs, int(n[s]), fc[s])  # plan, untimed
+        gi.sync()
+        t0 = time.perf_counter()
+        for _ in range(seq_steps):
+            gi.force_collision(Ps, Vs, Fs, int(n[s]), fc[s])
+            gi.move_collision(Ps, Vs, Fs, int(n[s]), mc[s])
+        gi.sync()
+        t_plain += time.perf_counter() - t0
     dE = np.abs(c1[:, 2] - c0[:, 2]) / np.abs(c0[:, 2])
     dL = np.linalg.norm(c1[:, 6:9] - c0[:, 6:9], axis=1) / scaleL
     batched_rate = n_sims * steps / t_batched
     seq_rate = n_sims * seq_steps / t_seq
+    plain_rate = n_sims * seq_steps / t_plain
     return {
         "sims": n_sims,
         "particles": ntot,
@@ -146,8 +165,10 @@ def ensemble_leg(gi, n_sims: int, steps: int, seq_steps: int = 100, check_every:
         "steps": steps,
         "sequential_steps": seq_steps,
         "batched_sim_steps_per_s": batched_rate,
-        "sequential_sim_steps_per_s": seq_rate,
-        "speedup_vs_sequential": batched_rate / seq_rate,
+        "sequential_plain_kernel_sim_steps_per_s": plain_rate,
+        "speedup_vs_sequential": batched_rate / plain_rate,
+        "sequential_graph_stepper_sim_steps_per_s": seq_rate,
+        "speedup_vs_sequential_graph_stepper": batched_rate / seq_rate,
         "batched_pair_per_s": pairs * steps / t_batched,
         "fp64_check_every": check_every,
         "fp64_energy_rel_change_max": float(dE.max()),
@@ -280,16 +301,30 @@ def drift_leg(gi, steps_small: int, steps_large: int, check_every: int, steps_he
     po, vo, _ = O.impact_steps_f32(pos, vel, fc, mc, steps_small)
     t_ref = time.perf_counter() - t0
     r1 = O.conserved_f64(po, vo, fc)
+    # the faithful single-thread form, on the same steps
+    t0 = time.perf_counter()
+    po1, vo1, _ = O.impact_steps_f32(pos, vel, fc, mc, steps_small, nthreads=1)
+    t_ref1 = time.perf_counter() - t0
     pairs = float(n) * n * steps_small
     out["n2048"] = {"steps": steps_small, "gpu": rel(c0, g1), "reference_float_oracle": rel(c0, r1),
                     "gpu_seconds": t_gpu, "gpu_pair_per_s": pairs / t_gpu,
                     "oracle_seconds": t_ref, "oracle_pair_per_s": pairs / t_ref,
-                    "oracle_threads": int(O.threads())}
+                    "oracle_threads": int(O.threads()),
+                    "oracle_1thread_seconds": t_ref1, "oracle_1thread_pair_per_s": pairs / t_ref1,
+                    "oracle_threads_agree": bool(np.array_equal(po, po1) and np.array_equal(vo, vo1))}
     pos, vel, fc, mc, _ = gi.make_impact(52429, 13107, seed=20261203, angle_deg=45.0, speed_vesc=1.1,
                                          target_spin=0.5)
     n = len(pos)
     P, V, F = gi.DeviceArray(n, pos), gi.DeviceArray(n, vel), gi.DeviceArray(n, np.zeros((n, 4), np.float32))
     step1 = spot_parity(gi, pos, vel, fc, P, V, F)  # the first step's force against FP64
+    cpu_steps = 3
+    t0 = time.perf_counter()
+    O.impact_steps_f32(pos, vel, fc, mc, cpu_steps)
+    t_cpu = (time.perf_counter() - t0) / cpu_steps
+    cpu65 = {"steps_timed": cpu_steps, "threads": int(O.threads()), "ms_per_step": t_cpu * 1e3,
+             "pair_per_s": float(n) * n / t_cpu,
+             "extrapolated_s_for_all_steps": t_cpu * steps_large,
+             "note": f"extrapolated (O(N^2) work per step) from {cpu_steps} timed steps of the reference-float path"}
     c0 = gi.conserved_quantities(P, V, n, fc)
     series = []
     t0 = time.perf_counter()
@@ -299,6 +334,7 @@ def drift_leg(gi, steps_small: int, steps_large: int, check_every: int, steps_he
         series.append(rel(c0, c))
     wall = time.perf_counter() - t0
     out["n65536"] = {"steps": steps_large, "check_every": check_every, "wall_s": wall, "step1_force_parity": step1,
+                     "reference_float_oracle": cpu65,
                      "final": series[-1],
                      "angmom_rel_change_max": max(x["angmom_rel_change"] for x in series),
                      "energy_rel_change_series": [x["energy_rel_change"] for x in series]}
