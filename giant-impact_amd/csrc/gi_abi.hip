@@ -291,12 +291,19 @@ static int plan_mutual(std::vector<SimDesc>& sims, const Tiling& tl, bool batche
         const SimDesc& s = sims[si];
         if (s.gn == 0) continue;
         const int nrows = s.gn / B;
+        // End of each segment's last 64-block holding a real particle: j
+        // beyond it are ghosts only, whose terms are exactly zero, so no task
+        // streams them (the rows themselves keep their ghost slots).
+        const int fe_real = (s.fe_end - s.start + 63) / 64 * 64;
+        const int si_real = s.gfe + (s.end - s.fe_end + 63) / 64 * 64;
         for (int I = 0; I < nrows; ++I) {
             RowDesc rd{slot, 0};
             const int rb = I * B;
-            tasks.push_back(MutualTask{(int)si, I, rb, rb + B, slot++, rb < s.gfe ? 1 : 0, 1, 0});
+            const bool rowfe = rb < s.gfe;
+            tasks.push_back(
+                MutualTask{(int)si, I, rb, std::min(rb + B, rowfe ? fe_real : si_real), slot++, rowfe ? 1 : 0, 1, 0});
             const int seg0[2] = {rb + B, std::max(rb + B, s.gfe)};
-            const int seg1[2] = {s.gfe, s.gn};
+            const int seg1[2] = {fe_real, si_real};
             for (int g = 0; g < 2; ++g) {
                 const int a0 = seg0[g], a1 = seg1[g];
                 if (a1 <= a0) continue;
