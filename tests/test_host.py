@@ -5,6 +5,7 @@ from __future__ import annotations
 import ctypes as C
 import re
 import subprocess
+import sys
 
 import numpy as np
 import pytest
@@ -229,3 +230,12 @@ def test_every_tuning_row_is_accepted(gi):
         assert gi.tiling_for(1000)["variant"] == 3
     finally:
         gi.call("gi_set_tiling", 0, 0, 0, 0, 0)
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU: --gpus 2 exits with a message before it
+    touches a device (the contract's multi-GPU launch gets a clear answer)."""
+    r = subprocess.run([sys.executable, str(REPO / "bench.py"), "--gpus", "2"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr and r.stdout == ""
