@@ -123,12 +123,28 @@ __global__ __launch_bounds__(256) void k_cc_union(const float4* __restrict__ pos
             }
 }
 
-// root[i] = the smallest index of i's component; size[root]++.
+// Root of x without path compression: after the unions the forest is fixed,
+// and the only stores to parent are each thread's own final root.
+__device__ __forceinline__ int cc_root(const int* parent, int x) {
+    int p = cc_load(parent + x);
+    while (p != x) {
+        x = p;
+        p = cc_load(parent + x);
+    }
+    return x;
+}
+
+// root[i] = the smallest index of i's component; size[root]++.  The walk
+// must not compress paths: a path-splitting store from another thread's walk
+// (a grandparent, not necessarily the root) could land after i's own root
+// store and leave parent[i] pointing at an inner node.  Reading a parent
+// that another thread has just replaced by its root is harmless: both are
+// ancestors.
 __global__ void k_cc_compress(int* __restrict__ parent, int* __restrict__ size, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const int r = cc_find(parent, i);
-    parent[i] = r;
+    const int r = cc_root(parent, i);
+    __hip_atomic_store(parent + i, r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     atomicAdd(&size[r], 1);
 }
 

@@ -125,3 +125,27 @@ def test_invalid_arguments(gpu):
     assert lib.gi_find_earth_moon(P.ptr, V.ptr, 8, 5, 5, 0, 2.0, 1.0, 1.5, None, C.byref(em), None) == -1
     assert lib.gi_find_earth_moon(P.ptr, V.ptr, 8, 1, 1, 1, 2.0, 1.0, 0.0, None, C.byref(em), None) == -1
     assert lib.gi_find_earth_moon(P.ptr, V.ptr, 8, 1, 1, 1, 2.0, 1.0, 1.5, None, None, None) == -1
+
+
+def test_long_chains_with_shuffled_indices_are_one_group_each(gpu):
+    """Deep union-find trees: 8 chains of 4,000 elements 1.4 apart (touch 1.5),
+    3 apart across chains, indices shuffled.  Every element's group is the
+    smallest index of its chain -- the final root pointers are never left on
+    an inner node of a tree."""
+    rng = np.random.default_rng(11)
+    chains, per = 8, 4000
+    n = chains * per
+    pts = np.zeros((n, 3), np.float32)
+    chain_of = np.repeat(np.arange(chains), per)
+    pts[:, 0] = np.tile(np.arange(per, dtype=np.float32) * np.float32(1.4), chains)
+    pts[:, 1] = chain_of.astype(np.float32) * np.float32(3.0)
+    perm = rng.permutation(n)
+    pos, vel = f4(n), f4(n)
+    pos[perm, :3] = pts
+    chain_of = chain_of[np.argsort(perm)]  # chain of each (shuffled) index
+    P, V = gpu.DeviceArray(n, pos), gpu.DeviceArray(n, vel)
+    g = gpu.find_earth_moon(P, V, n, 0, 0, n, 2.364458, with_groups=True)
+    want = np.array([np.flatnonzero(chain_of == c).min() for c in range(chains)])[chain_of]
+    assert g["n_groups"] == chains
+    assert np.array_equal(g["group"], want)
+    assert g["earth_count"] == per and g["moon_count"] == per
