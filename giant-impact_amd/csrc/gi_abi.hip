@@ -109,6 +109,9 @@ struct Plan {
     long long iside_len = 0;  // floats
     long long jside_len = 0;  // floats
     long long chunk_len = 0;  // j per task of the mutual pass (0 if none)
+    // every simulation takes the mutual pass: the contact corrections run
+    // beside it (k_contact_corr) and k_mutual_sum writes the force
+    bool split_finish = false;
     // direct pass (tiling.variant == 3): one workgroup per 64 / direct_lanes
     // i-particles
     bool direct = false;
@@ -127,6 +130,7 @@ struct Workspace {
     DevBuf<DirectBlock> dblocks;
     DevBuf<int> counts, starts, sorted, cellhash, block_sums;
     DevBuf<float4> spos, svel;  // positions / velocities in bucket order
+    DevBuf<double> corr;        // contact corrections (3 planes of ntot)
     DevBuf<GravTask> tasks;
     DevBuf<SimDesc> sims;
     DevBuf<int> block_sim, block_first, sim_first_block;
@@ -437,6 +441,7 @@ static int build_plan(std::vector<SimDesc> sims, const Tiling& tl, bool batched,
         }
         p.sim_first_block.push_back((int)p.block_sim.size());
     }
+    p.split_finish = p.mutual && p.tasks.empty() && !p.direct;
     p.sims = std::move(sims);
     *out = std::move(p);
     return 0;
@@ -473,6 +478,7 @@ static int ensure_plan(Workspace* ws, const std::vector<SimDesc>& sims, hipStrea
         if ((rc = ws->jside.ensure((size_t)p.jside_len + 1))) return rc;
         if ((rc = upload(ws->mtasks, p.mtasks, st))) return rc;
         if ((rc = upload(ws->rows, p.rows, st))) return rc;
+        if (p.split_finish && (rc = ws->corr.ensure((size_t)3 * p.ntot + 1))) return rc;
     }
     if (p.direct && (rc = upload(ws->dblocks, p.dblocks, st))) return rc;
     if ((rc = ws->counts.ensure((size_t)p.buckets + 1))) return rc;
@@ -587,13 +593,18 @@ static void launch_mutual_kernel(int R, hipStream_t st, const Workspace* ws, int
 // The all-pairs pass of the current plan: the one-sided kernel for the
 // simulations planned one-sided, and for the others the mutual pass (padded
 // copy, pair kernel, per-particle sums into ws->gsum).  Timed: kind 0 spans
-// the all-pairs kernel(s), kind 2 the whole pass.
+// the all-pairs kernel(s), kind 2 the whole pass.  With force (plans with
+// split_finish), st waits for `join` (the contact corrections in ws->corr)
+// and the sums go to force with the corrections added.
 static int launch_gravity(DevCtx* ctx, Workspace* ws, const float4* pos, bool pot, hipStream_t st,
-                          bool timed) {
+                          bool timed, float4* force = nullptr, hipEvent_t join = nullptr) {
     const Plan& p = ws->plan;
     const int nplain = (int)p.tasks.size();
     const int nmut = (int)p.mtasks.size();
-    if (nplain == 0 && nmut == 0) return 0;
+    if (nplain == 0 && nmut == 0) {
+        if (join) GI_HIP(hipStreamWaitEvent(st, join, 0));
+        return 0;
+    }
     const int nsims = (int)p.sims.size();
     TimingPair pass, tp;
     if (timed) {
@@ -633,9 +644,11 @@ static int launch_gravity(DevCtx* ctx, Workspace* ws, const float4* pos, bool po
         GI_HIP(hipEventRecord(tp.b, st));
         ctx->pending.push_back(tp);
     }
+    if (join) GI_HIP(hipStreamWaitEvent(st, join, 0));
     if (nmut > 0) {
         hipLaunchKernelGGL(k_mutual_sum, dim3(nblk(p.ntot, 64)), dim3(64 * kSumSplit), 0, st, p.ntot, ws->sims.p,
-                           nsims, ws->rows.p, 64 * p.tiling.R, ws->iside.p, ws->jside.p, ws->gsum.p, pot ? 1 : 0);
+                           nsims, ws->rows.p, 64 * p.tiling.R, ws->iside.p, ws->jside.p, ws->gsum.p, pot ? 1 : 0,
+                           force ? (const double*)ws->corr.p : nullptr, force);
         GI_HIP(hipGetLastError());
     }
     if (timed) {
@@ -708,21 +721,33 @@ static int run_force(DevCtx* ctx, Workspace* ws, const float4* pos, const float4
         if ((rc = launch_direct(ctx, ws, pos, vel, force, false, st, timed))) return rc;
     } else {
         // The contact grid reads only pos / vel: it is built on the side
-        // stream while the all-pairs pass runs on st, and joined before
-        // k_finish (under graph capture the fork and join become edges).
+        // stream while the all-pairs pass runs on st (under graph capture the
+        // fork and join become edges).  When every simulation takes the
+        // mutual pass, the contact corrections follow on the side stream and
+        // k_mutual_sum adds them; otherwise k_finish adds them after the join.
         if (!ctx->side) GI_HIP(hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking));
         if (!ctx->ev_fork) GI_HIP(hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));
         if (!ctx->ev_join) GI_HIP(hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming));
         GI_HIP(hipEventRecord(ctx->ev_fork, st));
         GI_HIP(hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
         if ((rc = launch_grid(ws, pos, vel, ctx->side))) return rc;
+        if (p.split_finish) {
+            hipLaunchKernelGGL(k_contact_corr, dim3(nblk(p.ntot, 256)), dim3(256), 0, ctx->side, p.ntot,
+                               ws->sims.p, (int)p.sims.size(), ws->starts.p, ws->sorted.p, ws->spos.p, ws->svel.p,
+                               p.tiling.variant, ws->corr.p);
+            GI_HIP(hipGetLastError());
+        }
         GI_HIP(hipEventRecord(ctx->ev_join, ctx->side));
-        if ((rc = launch_gravity(ctx, ws, pos, false, st, timed))) return rc;
-        GI_HIP(hipStreamWaitEvent(st, ctx->ev_join, 0));
-        hipLaunchKernelGGL(k_finish, dim3(nblk(p.ntot, 256)), dim3(256), 0, st, force, p.ntot, ws->sims.p,
-                           (int)p.sims.size(), ws->partial.p, p.mutual ? ws->gsum.p : nullptr, ws->starts.p,
-                           ws->sorted.p, ws->spos.p, ws->svel.p, p.tiling.variant);
-        GI_HIP(hipGetLastError());
+        if (p.split_finish) {
+            if ((rc = launch_gravity(ctx, ws, pos, false, st, timed, force, ctx->ev_join))) return rc;
+        } else {
+            if ((rc = launch_gravity(ctx, ws, pos, false, st, timed))) return rc;
+            GI_HIP(hipStreamWaitEvent(st, ctx->ev_join, 0));
+            hipLaunchKernelGGL(k_finish, dim3(nblk(p.ntot, 256)), dim3(256), 0, st, force, p.ntot, ws->sims.p,
+                               (int)p.sims.size(), ws->partial.p, p.mutual ? ws->gsum.p : nullptr, ws->starts.p,
+                               ws->sorted.p, ws->spos.p, ws->svel.p, p.tiling.variant);
+            GI_HIP(hipGetLastError());
+        }
     }
     if (timed) {
         GI_HIP(hipEventRecord(whole.b, st));

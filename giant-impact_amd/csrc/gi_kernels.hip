@@ -351,13 +351,18 @@ __global__ void k_mutual_pad(const float4* __restrict__ pos, int gtot, const Sim
 // each particle's slots and slab entries (one particle per lane, so the loads
 // stay coalesced), and the quarters are added in LDS in a fixed order --
 // four times the loads in flight of one thread per particle.
+// With corr (the contact corrections of k_contact_corr), the sum rounded to
+// float plus the correction goes to force.xyz -- k_finish's arithmetic;
+// without, the sum goes to gsum for k_finish / k_finish_potential.
 constexpr int kSumSplit = 4;
 
 __global__ __launch_bounds__(64 * kSumSplit) void k_mutual_sum(int ntot, const SimDesc* __restrict__ sims,
                                                               int nsims, const RowDesc* __restrict__ rows, int B,
                                                               const float* __restrict__ iside,
                                                               const float* __restrict__ jside,
-                                                              float4* __restrict__ gsum, int pot) {
+                                                              float4* __restrict__ gsum, int pot,
+                                                              const double* __restrict__ corr,
+                                                              float4* __restrict__ force) {
     __shared__ double red[3][kSumSplit][64];
     const int lane = threadIdx.x & 63;
     const int w = threadIdx.x >> 6;
@@ -403,7 +408,13 @@ __global__ __launch_bounds__(64 * kSumSplit) void k_mutual_sum(int ntot, const S
             ty += red[1][k][lane];
             tz += red[2][k][lane];
         }
-        gsum[i] = make_float4((float)tx, (float)ty, (float)tz, 0.0f);
+        if (corr) {
+            force[i].x = (float)((double)(float)tx + corr[i]);
+            force[i].y = (float)((double)(float)ty + corr[ntot + i]);
+            force[i].z = (float)((double)(float)tz + corr[2 * ntot + i]);
+        } else {
+            gsum[i] = make_float4((float)tx, (float)ty, (float)tz, 0.0f);
+        }
     }
 }
 
@@ -534,58 +545,30 @@ __global__ void k_grid_gather(int ntot, const int* __restrict__ sorted, const fl
 // Pass 3: chunk sums + exact contact correction
 // ---------------------------------------------------------------------------
 
-__global__ __launch_bounds__(256) void k_finish(float4* __restrict__ force, int ntot,
-                                                const SimDesc* __restrict__ sims, int nsims,
-                                                const float4* __restrict__ partial,
-                                                const float4* __restrict__ gsum,
-                                                const int* __restrict__ starts,
-                                                const int* __restrict__ sorted,
-                                                const float4* __restrict__ spos,
-                                                const float4* __restrict__ svel, int variant) {
-    // Threads walk the particles in bucket (Morton) order: the lanes of a
-    // wavefront look up the same and adjacent cells.
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= ntot) return;
-    const int i = sorted[t];
-    const int s = find_sim(sims, nsims, i);
-    const SimDesc& sd = sims[s];
-    if (!sd.active) return;
-    const int n = sd.end - sd.start;
-
-    float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-    if (gsum && sd.gn > 0) {
-        // mutual pass: already summed per particle
-        const float4 gs = gsum[i];
-        fx = gs.x;
-        fy = gs.y;
-        fz = gs.z;
-    } else {
-        float kx = 0.0f, ky = 0.0f, kz = 0.0f;
-        for (int c = 0; c < sd.nchunk; ++c) {
-            float4 p = partial[sd.partial_base + c * n + (i - sd.start)];
-            kahan_add(fx, kx, p.x);
-            kahan_add(fy, ky, p.y);
-            kahan_add(fz, kz, p.z);
-        }
-        fx -= kx;
-        fy -= ky;
-        fz -= kz;
-    }
-
+// The contact correction of particle i (at bucket position t of simulation
+// sd): for every pair with r < 1 found through the grid, the reference's
+// exact contact term minus what the all-pairs pass added for that pair.
+// Contact terms are rounded exactly as the reference rounds them (float
+// products of a float magnitude); only their sum is carried in FP64, so
+// cancellation between large contact forces adds no summation error.  Cells
+// are visited in a fixed order and a bucket lists its particles by index, so
+// the order of the terms does not depend on the rest of the launch.  A
+// simulation's buckets hold only its own particles, and i itself sits at
+// position t, so a candidate's index is loaded only for a pair in contact;
+// the three bucket ranges of a row of cells are loaded together.
+__device__ __forceinline__ void contact_correction(const SimDesc& sd, int t, int i,
+                                                   const int* __restrict__ starts,
+                                                   const int* __restrict__ sorted,
+                                                   const float4* __restrict__ spos,
+                                                   const float4* __restrict__ svel, int variant, double& ex,
+                                                   double& ey, double& ez) {
     const float4 pi = spos[t];
     const float4 vi = svel[t];
     const int ccx = cell_coord(pi.x), ccy = cell_coord(pi.y), ccz = cell_coord(pi.z);
     const bool ife = i < sd.fe_end;
-    // Contact terms are rounded exactly as the reference rounds them (float
-    // products of a float magnitude); only their sum is carried in FP64, so
-    // cancellation between large contact forces adds no summation error.
-    // Cells are visited in a fixed order and a bucket lists its particles by
-    // index, so the order of the terms does not depend on the rest of the
-    // launch.  A simulation's buckets hold only its own particles, and i
-    // itself sits at position t, so a candidate's index is loaded only for a
-    // pair in contact; the three bucket ranges of a row of cells are loaded
-    // together.
-    double ex = 0.0, ey = 0.0, ez = 0.0;
+    ex = 0.0;
+    ey = 0.0;
+    ez = 0.0;
     for (int oz = -1; oz <= 1; ++oz)
         for (int oy = -1; oy <= 1; ++oy) {
             const int qy = ccy + oy, qz = ccz + oz;
@@ -625,9 +608,72 @@ __global__ __launch_bounds__(256) void k_finish(float4* __restrict__ force, int 
                 }
             }
         }
+}
+
+__global__ __launch_bounds__(256) void k_finish(float4* __restrict__ force, int ntot,
+                                                const SimDesc* __restrict__ sims, int nsims,
+                                                const float4* __restrict__ partial,
+                                                const float4* __restrict__ gsum,
+                                                const int* __restrict__ starts,
+                                                const int* __restrict__ sorted,
+                                                const float4* __restrict__ spos,
+                                                const float4* __restrict__ svel, int variant) {
+    // Threads walk the particles in bucket (Morton) order: the lanes of a
+    // wavefront look up the same and adjacent cells.
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntot) return;
+    const int i = sorted[t];
+    const int s = find_sim(sims, nsims, i);
+    const SimDesc& sd = sims[s];
+    if (!sd.active) return;
+    const int n = sd.end - sd.start;
+
+    float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+    if (gsum && sd.gn > 0) {
+        // mutual pass: already summed per particle
+        const float4 gs = gsum[i];
+        fx = gs.x;
+        fy = gs.y;
+        fz = gs.z;
+    } else {
+        float kx = 0.0f, ky = 0.0f, kz = 0.0f;
+        for (int c = 0; c < sd.nchunk; ++c) {
+            float4 p = partial[sd.partial_base + c * n + (i - sd.start)];
+            kahan_add(fx, kx, p.x);
+            kahan_add(fy, ky, p.y);
+            kahan_add(fz, kz, p.z);
+        }
+        fx -= kx;
+        fy -= ky;
+        fz -= kz;
+    }
+    double ex, ey, ez;
+    contact_correction(sd, t, i, starts, sorted, spos, svel, variant, ex, ey, ez);
     force[i].x = (float)((double)fx + ex);
     force[i].y = (float)((double)fy + ey);
     force[i].z = (float)((double)fz + ez);
+}
+
+// The contact corrections alone, into corr (three planes of ntot doubles,
+// by particle index), for plans whose simulations all take the mutual pass:
+// it needs only the grid, so it runs beside the all-pairs pass, and
+// k_mutual_sum adds it to the sums -- the same arithmetic as k_finish.
+__global__ __launch_bounds__(256) void k_contact_corr(int ntot, const SimDesc* __restrict__ sims, int nsims,
+                                                      const int* __restrict__ starts,
+                                                      const int* __restrict__ sorted,
+                                                      const float4* __restrict__ spos,
+                                                      const float4* __restrict__ svel, int variant,
+                                                      double* __restrict__ corr) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntot) return;
+    const int i = sorted[t];
+    const SimDesc& sd = sims[find_sim(sims, nsims, i)];
+    if (!sd.active) return;
+    double ex, ey, ez;
+    contact_correction(sd, t, i, starts, sorted, spos, svel, variant, ex, ey, ez);
+    corr[i] = ex;
+    corr[ntot + i] = ey;
+    corr[2 * ntot + i] = ez;
 }
 
 // ---------------------------------------------------------------------------
