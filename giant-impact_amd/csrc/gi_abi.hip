@@ -12,6 +12,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <initializer_list>
 #include <mutex>
 #include <string>
@@ -105,6 +106,7 @@ struct Plan {
     bool mutual = false;
     std::vector<MutualTask> mtasks;
     std::vector<RowDesc> rows;
+    std::vector<SumBlock> sblocks;  // k_mutual_sum's workgroups
     long long gtot = 0;       // padded particles
     long long iside_len = 0;  // floats
     long long jside_len = 0;  // floats
@@ -127,6 +129,7 @@ struct Workspace {
     DevBuf<float> iside, jside;    // mutual pass: row slots, j-side slabs
     DevBuf<MutualTask> mtasks;
     DevBuf<RowDesc> rows;
+    DevBuf<SumBlock> sblocks;
     DevBuf<DirectBlock> dblocks;
     DevBuf<int> counts, starts, sorted, cellhash, block_sums;
     DevBuf<float4> spos, svel;  // positions / velocities in bucket order
@@ -324,6 +327,27 @@ static int plan_mutual(std::vector<SimDesc>& sims, const Tiling& tl, bool batche
             rows.push_back(rd);
         }
     }
+    // k_mutual_sum's workgroups: the 64-blocks of every row that hold a real
+    // particle, heaviest first (a particle adds its row's slots and one slab
+    // entry per earlier row).
+    std::vector<SumBlock> sblocks;
+    std::vector<int> swork;
+    for (size_t si = 0; si < sims.size(); ++si) {
+        const SimDesc& s = sims[si];
+        if (s.gn == 0) continue;
+        const int nfe = s.fe_end - s.start, nsi = s.end - s.fe_end;
+        for (int q0 = 0; q0 < s.gn; q0 += 64) {
+            if (q0 < s.gfe ? q0 >= nfe : q0 - s.gfe >= nsi) continue;  // ghosts only
+            const int I = q0 / B;
+            sblocks.push_back(SumBlock{(int)si, q0});
+            swork.push_back(rows[s.row0 + I].nslot + I);
+        }
+    }
+    std::vector<int> order(sblocks.size());
+    for (size_t k = 0; k < order.size(); ++k) order[k] = (int)k;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return swork[a] > swork[b]; });
+    p.sblocks.resize(order.size());
+    for (size_t k = 0; k < order.size(); ++k) p.sblocks[k] = sblocks[order[k]];
     // Longest first (VALU ops per lane: R*12 per j one-sided, R*15+6 mutual).
     auto cost = [&](const MutualTask& t) {
         return (long long)(t.j1 - t.j0) * (t.diag ? 12 * tl.R : 15 * tl.R + 6);
@@ -478,6 +502,7 @@ static int ensure_plan(Workspace* ws, const std::vector<SimDesc>& sims, hipStrea
         if ((rc = ws->jside.ensure((size_t)p.jside_len + 1))) return rc;
         if ((rc = upload(ws->mtasks, p.mtasks, st))) return rc;
         if ((rc = upload(ws->rows, p.rows, st))) return rc;
+        if ((rc = upload(ws->sblocks, p.sblocks, st))) return rc;
         if (p.split_finish && (rc = ws->corr.ensure((size_t)3 * p.ntot + 1))) return rc;
     }
     if (p.direct && (rc = upload(ws->dblocks, p.dblocks, st))) return rc;
@@ -595,13 +620,18 @@ static void launch_mutual_kernel(int R, hipStream_t st, const Workspace* ws, int
 // copy, pair kernel, per-particle sums into ws->gsum).  Timed: kind 0 spans
 // the all-pairs kernel(s), kind 2 the whole pass.  With force (plans with
 // split_finish), st waits for `join` (the contact corrections in ws->corr)
-// and the sums go to force with the corrections added.
+// and the sums go to force with the corrections added.  after_pad runs right
+// after the padded copy is enqueued (work forked from there does not compete
+// with the copy).
 static int launch_gravity(DevCtx* ctx, Workspace* ws, const float4* pos, bool pot, hipStream_t st,
-                          bool timed, float4* force = nullptr, hipEvent_t join = nullptr) {
+                          bool timed, float4* force = nullptr, hipEvent_t join = nullptr,
+                          const std::function<int()>& after_pad = nullptr) {
     const Plan& p = ws->plan;
     const int nplain = (int)p.tasks.size();
     const int nmut = (int)p.mtasks.size();
+    int rc;
     if (nplain == 0 && nmut == 0) {
+        if (after_pad && (rc = after_pad())) return rc;
         if (join) GI_HIP(hipStreamWaitEvent(st, join, 0));
         return 0;
     }
@@ -614,6 +644,7 @@ static int launch_gravity(DevCtx* ctx, Workspace* ws, const float4* pos, bool po
     if (nmut > 0)
         hipLaunchKernelGGL(k_mutual_pad, dim3(nblk(p.gtot, 256)), dim3(256), 0, st, pos, (int)p.gtot, ws->sims.p,
                            nsims, ws->gpos.p);
+    if (after_pad && (rc = after_pad())) return rc;
     if (timed) {
         tp = take_pair(ctx, 0);
         GI_HIP(hipEventRecord(tp.a, st));
@@ -645,10 +676,10 @@ static int launch_gravity(DevCtx* ctx, Workspace* ws, const float4* pos, bool po
         ctx->pending.push_back(tp);
     }
     if (join) GI_HIP(hipStreamWaitEvent(st, join, 0));
-    if (nmut > 0) {
-        hipLaunchKernelGGL(k_mutual_sum, dim3(nblk(p.ntot, 64)), dim3(64 * kSumSplit), 0, st, p.ntot, ws->sims.p,
-                           nsims, ws->rows.p, 64 * p.tiling.R, ws->iside.p, ws->jside.p, ws->gsum.p, pot ? 1 : 0,
-                           force ? (const double*)ws->corr.p : nullptr, force);
+    if (nmut > 0 && !p.sblocks.empty()) {
+        hipLaunchKernelGGL(k_mutual_sum, dim3((unsigned)p.sblocks.size()), dim3(64 * kSumSplit), 0, st,
+                           ws->sblocks.p, ws->sims.p, ws->rows.p, 64 * p.tiling.R, ws->iside.p, ws->jside.p,
+                           ws->gsum.p, pot ? 1 : 0, force ? (const double*)ws->corr.p : nullptr, p.ntot, force);
         GI_HIP(hipGetLastError());
     }
     if (timed) {
@@ -725,23 +756,29 @@ static int run_force(DevCtx* ctx, Workspace* ws, const float4* pos, const float4
         // fork and join become edges).  When every simulation takes the
         // mutual pass, the contact corrections follow on the side stream and
         // k_mutual_sum adds them; otherwise k_finish adds them after the join.
+        // The fork follows the mutual pass's padded copy, which then has the
+        // device to itself.
         if (!ctx->side) GI_HIP(hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking));
         if (!ctx->ev_fork) GI_HIP(hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));
         if (!ctx->ev_join) GI_HIP(hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming));
-        GI_HIP(hipEventRecord(ctx->ev_fork, st));
-        GI_HIP(hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
-        if ((rc = launch_grid(ws, pos, vel, ctx->side))) return rc;
+        const std::function<int()> fork = [&]() -> int {
+            int r;
+            GI_HIP(hipEventRecord(ctx->ev_fork, st));
+            GI_HIP(hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
+            if ((r = launch_grid(ws, pos, vel, ctx->side))) return r;
+            if (p.split_finish) {
+                hipLaunchKernelGGL(k_contact_corr, dim3(nblk(p.ntot, 256)), dim3(256), 0, ctx->side, p.ntot,
+                                   ws->sims.p, (int)p.sims.size(), ws->starts.p, ws->sorted.p, ws->spos.p,
+                                   ws->svel.p, p.tiling.variant, ws->corr.p);
+                GI_HIP(hipGetLastError());
+            }
+            GI_HIP(hipEventRecord(ctx->ev_join, ctx->side));
+            return 0;
+        };
         if (p.split_finish) {
-            hipLaunchKernelGGL(k_contact_corr, dim3(nblk(p.ntot, 256)), dim3(256), 0, ctx->side, p.ntot,
-                               ws->sims.p, (int)p.sims.size(), ws->starts.p, ws->sorted.p, ws->spos.p, ws->svel.p,
-                               p.tiling.variant, ws->corr.p);
-            GI_HIP(hipGetLastError());
-        }
-        GI_HIP(hipEventRecord(ctx->ev_join, ctx->side));
-        if (p.split_finish) {
-            if ((rc = launch_gravity(ctx, ws, pos, false, st, timed, force, ctx->ev_join))) return rc;
+            if ((rc = launch_gravity(ctx, ws, pos, false, st, timed, force, ctx->ev_join, fork))) return rc;
         } else {
-            if ((rc = launch_gravity(ctx, ws, pos, false, st, timed))) return rc;
+            if ((rc = launch_gravity(ctx, ws, pos, false, st, timed, nullptr, nullptr, fork))) return rc;
             GI_HIP(hipStreamWaitEvent(st, ctx->ev_join, 0));
             hipLaunchKernelGGL(k_finish, dim3(nblk(p.ntot, 256)), dim3(256), 0, st, force, p.ntot, ws->sims.p,
                                (int)p.sims.size(), ws->partial.p, p.mutual ? ws->gsum.p : nullptr, ws->starts.p,

@@ -55,6 +55,12 @@ struct RowDesc {
     int slot0, nslot;
 };
 
+// One workgroup of k_mutual_sum: the 64 padded places [q0, q0 + 64) of
+// simulation sim (inside one row, as 64 divides the row length).
+struct SumBlock {
+    int sim, q0;
+};
+
 // Ghost position of the padded layout: every pair with a real particle has
 // r^-3 = 0 in float (r^2 ~ 3e36), and a ghost pair has dp = 0.
 constexpr float kGhostCoord = 1e18f;

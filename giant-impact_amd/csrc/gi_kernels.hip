@@ -345,54 +345,60 @@ __global__ void k_mutual_pad(const float4* __restrict__ pos, int gtot, const Sim
     gpos[q] = src >= 0 ? pos[src] : make_float4(kGhostCoord, kGhostCoord, kGhostCoord, 0.0f);
 }
 
-// Per particle (index order): the i-side slots of its row and the j-side
-// entries of every earlier row, summed in FP64 in a fixed order.
-// Four wavefronts per 64 particles: wavefront w adds the w-th quarter of
-// each particle's slots and slab entries (one particle per lane, so the loads
-// stay coalesced), and the quarters are added in LDS in a fixed order --
-// four times the loads in flight of one thread per particle.
+// Per particle: the i-side slots of its row and the j-side entries of every
+// earlier row, summed in FP64 in a fixed order.  A workgroup takes 64 padded
+// places of one row (a SumBlock; the host lists the blocks holding a real
+// particle, heaviest rows first), so the loop bounds are uniform.  Four
+// wavefronts per block: wavefront w adds the w-th quarter of each particle's
+// slots and slab entries (one particle per lane, so the loads stay
+// coalesced), and the quarters are added in LDS in a fixed order -- four
+// times the loads in flight of one thread per particle.
 // With corr (the contact corrections of k_contact_corr), the sum rounded to
 // float plus the correction goes to force.xyz -- k_finish's arithmetic;
 // without, the sum goes to gsum for k_finish / k_finish_potential.
 constexpr int kSumSplit = 4;
 
-__global__ __launch_bounds__(64 * kSumSplit) void k_mutual_sum(int ntot, const SimDesc* __restrict__ sims,
-                                                              int nsims, const RowDesc* __restrict__ rows, int B,
+__global__ __launch_bounds__(64 * kSumSplit) void k_mutual_sum(const SumBlock* __restrict__ blocks,
+                                                              const SimDesc* __restrict__ sims,
+                                                              const RowDesc* __restrict__ rows, int B,
                                                               const float* __restrict__ iside,
                                                               const float* __restrict__ jside,
                                                               float4* __restrict__ gsum, int pot,
-                                                              const double* __restrict__ corr,
+                                                              const double* __restrict__ corr, int ntot,
                                                               float4* __restrict__ force) {
     __shared__ double red[3][kSumSplit][64];
     const int lane = threadIdx.x & 63;
     const int w = threadIdx.x >> 6;
-    const int i = blockIdx.x * 64 + lane;
+    const SumBlock blk = blocks[blockIdx.x];
+    const SimDesc& sd = sims[blk.sim];
+    const int q = blk.q0 + lane;
+    const int row = blk.q0 / B;
+    const int o = q - row * B;
+    // the particle at padded place q, or -1 for a ghost
+    int i = -1;
+    if (q < sd.gfe) {
+        if (q < sd.fe_end - sd.start) i = sd.start + q;
+    } else if (q - sd.gfe < sd.end - sd.fe_end) {
+        i = sd.fe_end + (q - sd.gfe);
+    }
     double sx = 0.0, sy = 0.0, sz = 0.0;
-    bool live = false;
-    if (i < ntot) {
-        const SimDesc& sd = sims[find_sim(sims, nsims, i)];
-        live = sd.active && sd.gn > 0;  // gn == 0: one-sided pass
-        if (live) {
-            const int q = i < sd.fe_end ? i - sd.start : sd.gfe + (i - sd.fe_end);
-            const int row = q / B;
-            const int o = q - row * B;
-            const RowDesc rd = rows[sd.row0 + row];
-            for (int k = rd.nslot * w / kSumSplit; k < rd.nslot * (w + 1) / kSumSplit; ++k) {
-                const float* p = iside + (size_t)(rd.slot0 + k) * (3 * B) + o;
-                sx += (double)p[0];
-                if (!pot) {
-                    sy += (double)p[B];
-                    sz += (double)p[2 * B];
-                }
+    if (i >= 0) {
+        const RowDesc rd = rows[sd.row0 + row];
+        for (int k = rd.nslot * w / kSumSplit; k < rd.nslot * (w + 1) / kSumSplit; ++k) {
+            const float* p = iside + (size_t)(rd.slot0 + k) * (3 * B) + o;
+            sx += (double)p[0];
+            if (!pot) {
+                sy += (double)p[B];
+                sz += (double)p[2 * B];
             }
-            for (int r = row * w / kSumSplit; r < row * (w + 1) / kSumSplit; ++r) {
-                const long long len = sd.gn - (long long)(r + 1) * B;
-                const float* p = jside + sd.jbase + 3 * jside_row_offset(r, sd.gn, B) + (q - (r + 1) * B);
-                sx += (double)p[0];
-                if (!pot) {
-                    sy += (double)p[len];
-                    sz += (double)p[2 * len];
-                }
+        }
+        for (int r = row * w / kSumSplit; r < row * (w + 1) / kSumSplit; ++r) {
+            const long long len = sd.gn - (long long)(r + 1) * B;
+            const float* p = jside + sd.jbase + 3 * jside_row_offset(r, sd.gn, B) + (q - (r + 1) * B);
+            sx += (double)p[0];
+            if (!pot) {
+                sy += (double)p[len];
+                sz += (double)p[2 * len];
             }
         }
     }
@@ -400,7 +406,7 @@ __global__ __launch_bounds__(64 * kSumSplit) void k_mutual_sum(int ntot, const S
     red[1][w][lane] = sy;
     red[2][w][lane] = sz;
     __syncthreads();
-    if (w == 0 && live) {
+    if (w == 0 && i >= 0) {
         double tx = 0.0, ty = 0.0, tz = 0.0;
 #pragma unroll
         for (int k = 0; k < kSumSplit; ++k) {
