@@ -620,12 +620,13 @@ static void launch_mutual_kernel(int R, hipStream_t st, const Workspace* ws, int
 // copy, pair kernel, per-particle sums into ws->gsum).  Timed: kind 0 spans
 // the all-pairs kernel(s), kind 2 the whole pass.  With force (plans with
 // split_finish), st waits for `join` (the contact corrections in ws->corr)
-// and the sums go to force with the corrections added.  after_pad runs right
-// after the padded copy is enqueued (work forked from there does not compete
-// with the copy).
+// and the sums go to force with the corrections added (and with mpos / mvel
+// the batched move is applied).  after_pad runs right after the padded copy
+// is enqueued (work forked from there does not compete with the copy).
 static int launch_gravity(DevCtx* ctx, Workspace* ws, const float4* pos, bool pot, hipStream_t st,
                           bool timed, float4* force = nullptr, hipEvent_t join = nullptr,
-                          const std::function<int()>& after_pad = nullptr) {
+                          const std::function<int()>& after_pad = nullptr, float4* mpos = nullptr,
+                          float4* mvel = nullptr) {
     const Plan& p = ws->plan;
     const int nplain = (int)p.tasks.size();
     const int nmut = (int)p.mtasks.size();
@@ -679,7 +680,8 @@ static int launch_gravity(DevCtx* ctx, Workspace* ws, const float4* pos, bool po
     if (nmut > 0 && !p.sblocks.empty()) {
         hipLaunchKernelGGL(k_mutual_sum, dim3((unsigned)p.sblocks.size()), dim3(64 * kSumSplit), 0, st,
                            ws->sblocks.p, ws->sims.p, ws->rows.p, 64 * p.tiling.R, ws->iside.p, ws->jside.p,
-                           ws->gsum.p, pot ? 1 : 0, force ? (const double*)ws->corr.p : nullptr, p.ntot, force);
+                           ws->gsum.p, pot ? 1 : 0, force ? (const double*)ws->corr.p : nullptr, p.ntot, force, mpos,
+                           mvel);
         GI_HIP(hipGetLastError());
     }
     if (timed) {
@@ -737,9 +739,12 @@ static int launch_direct(DevCtx* ctx, Workspace* ws, const float4* pos, const fl
 }
 
 // Full force evaluation for the current plan: grid, all pairs, finish (or
-// the direct pass alone).
+// the direct pass alone).  With move_pos / move_vel (the arrays pos / vel
+// point to) and a plan with split_finish, the batched move (k_move) is fused
+// into the last kernel and *moved is set; otherwise the caller moves.
 static int run_force(DevCtx* ctx, Workspace* ws, const float4* pos, const float4* vel, float4* force,
-                     hipStream_t st, bool timed) {
+                     hipStream_t st, bool timed, float4* move_pos = nullptr, float4* move_vel = nullptr,
+                     bool* moved = nullptr) {
     const Plan& p = ws->plan;
     if (p.ntot == 0) return 0;
     TimingPair whole;
@@ -776,7 +781,11 @@ static int run_force(DevCtx* ctx, Workspace* ws, const float4* pos, const float4
             return 0;
         };
         if (p.split_finish) {
-            if ((rc = launch_gravity(ctx, ws, pos, false, st, timed, force, ctx->ev_join, fork))) return rc;
+            const bool fuse = move_pos && move_vel && moved;
+            if ((rc = launch_gravity(ctx, ws, pos, false, st, timed, force, ctx->ev_join, fork,
+                                     fuse ? move_pos : nullptr, fuse ? move_vel : nullptr)))
+                return rc;
+            if (fuse) *moved = true;
         } else {
             if ((rc = launch_gravity(ctx, ws, pos, false, st, timed, nullptr, nullptr, fork))) return rc;
             GI_HIP(hipStreamWaitEvent(st, ctx->ev_join, 0));
@@ -1642,10 +1651,13 @@ int gi_ensemble_steps(gi_ensemble* e, int n_steps) {
     hipGraphExec_t step;
     rc = get_graph(ctx, e->graphs, key, e->ws.generation,
                    [&](hipStream_t cs) -> int {
-                       int r = run_force(ctx, &e->ws, e->pos, e->vel, e->force.p, cs, false);
+                       bool moved = false;
+                       int r = run_force(ctx, &e->ws, e->pos, e->vel, e->force.p, cs, false, e->pos, e->vel,
+                                         &moved);
                        if (r) return r;
-                       hipLaunchKernelGGL(k_move, dim3(nblk(ntot, 256)), dim3(256), 0, cs, e->pos, e->vel,
-                                          (const float4*)e->force.p, ntot, e->ws.sims.p, ns);
+                       if (!moved)
+                           hipLaunchKernelGGL(k_move, dim3(nblk(ntot, 256)), dim3(256), 0, cs, e->pos, e->vel,
+                                              (const float4*)e->force.p, ntot, e->ws.sims.p, ns);
                        GI_HIP(hipGetLastError());
                        return 0;
                    },

@@ -354,8 +354,9 @@ __global__ void k_mutual_pad(const float4* __restrict__ pos, int gtot, const Sim
 // coalesced), and the quarters are added in LDS in a fixed order -- four
 // times the loads in flight of one thread per particle.
 // With corr (the contact corrections of k_contact_corr), the sum rounded to
-// float plus the correction goes to force.xyz -- k_finish's arithmetic;
-// without, the sum goes to gsum for k_finish / k_finish_potential.
+// float plus the correction goes to force.xyz -- k_finish's arithmetic --
+// and, with mpos / mvel, the particle then takes k_move's step; without
+// corr, the sum goes to gsum for k_finish / k_finish_potential.
 constexpr int kSumSplit = 4;
 
 __global__ __launch_bounds__(64 * kSumSplit) void k_mutual_sum(const SumBlock* __restrict__ blocks,
@@ -365,7 +366,9 @@ __global__ __launch_bounds__(64 * kSumSplit) void k_mutual_sum(const SumBlock* _
                                                               const float* __restrict__ jside,
                                                               float4* __restrict__ gsum, int pot,
                                                               const double* __restrict__ corr, int ntot,
-                                                              float4* __restrict__ force) {
+                                                              float4* __restrict__ force,
+                                                              float4* __restrict__ mpos,
+                                                              float4* __restrict__ mvel) {
     __shared__ double red[3][kSumSplit][64];
     const int lane = threadIdx.x & 63;
     const int w = threadIdx.x >> 6;
@@ -415,9 +418,26 @@ __global__ __launch_bounds__(64 * kSumSplit) void k_mutual_sum(const SumBlock* _
             tz += red[2][k][lane];
         }
         if (corr) {
-            force[i].x = (float)((double)(float)tx + corr[i]);
-            force[i].y = (float)((double)(float)ty + corr[ntot + i]);
-            force[i].z = (float)((double)(float)tz + corr[2 * ntot + i]);
+            const float fx = (float)((double)(float)tx + corr[i]);
+            const float fy = (float)((double)(float)ty + corr[ntot + i]);
+            const float fz = (float)((double)(float)tz + corr[2 * ntot + i]);
+            force[i].x = fx;
+            force[i].y = fy;
+            force[i].z = fz;
+            if (mpos) {
+                // k_move's arithmetic
+                const float temp = i < sd.fe_end ? sd.dt_over_mass_fe : sd.dt_over_mass_si;
+                float4 v = mvel[i];
+                float4 p = mpos[i];
+                v.x += fx * temp;
+                v.y += fy * temp;
+                v.z += fz * temp;
+                p.x += v.x * sd.dt;
+                p.y += v.y * sd.dt;
+                p.z += v.z * sd.dt;
+                mvel[i] = v;
+                mpos[i] = p;
+            }
         } else {
             gsum[i] = make_float4((float)tx, (float)ty, (float)tz, 0.0f);
         }
