@@ -119,3 +119,49 @@ def test_ensemble_conserved_quantities_match_single(gpu):
         # the potential's float chunk sums are cut differently by the two engines
         assert np.allclose(c[s][1:3], one[1:3], rtol=2e-6), s
     ens.close()
+
+
+def _run_in_calls(gi, pos, vel, n, fc, mc, calls):
+    P, V = gi.DeviceArray(len(pos), pos), gi.DeviceArray(len(pos), vel)
+    ens = gi.Ensemble(P, V, n, fc, mc)
+    for k in calls:
+        ens.steps(k)
+    gi.sync()
+    out = P.download(), V.download()
+    ens.close()
+    return out
+
+
+def test_step_calls_of_any_length_give_the_same_state(gpu):
+    """Within a call, steps after the first carry the padded positions from one
+    step to the next; the state must not depend on how the steps are cut into
+    calls."""
+    cfg = gpu.ensemble_configs(3, seed=8, n_range=(900, 3000))
+    pos, vel, n, fc, mc, _ = gpu.make_ensemble(cfg)
+    print("n =", list(n), flush=True)
+    ref = _run_in_calls(gpu, pos, vel, n, fc, mc, [10])
+    for calls in ([1] * 10, [3, 7], [2, 1, 5, 2]):
+        got = _run_in_calls(gpu, pos, vel, n, fc, mc, calls)
+        assert np.array_equal(got[0], ref[0]), calls
+        assert np.array_equal(got[1], ref[1]), calls
+
+
+def test_positions_changed_between_calls_are_used(gpu):
+    cfg = gpu.ensemble_configs(3, seed=9, n_range=(900, 3000))
+    pos, vel, n, fc, mc, _ = gpu.make_ensemble(cfg)
+    P, V = gpu.DeviceArray(len(pos), pos), gpu.DeviceArray(len(pos), vel)
+    ens = gpu.Ensemble(P, V, n, fc, mc)
+    ens.steps(4)
+    gpu.sync()
+    p, v = P.download(), V.download()
+    p[:, 0] += np.float32(0.25)  # shift every body; the caller owns the arrays
+    v[:, 1] *= np.float32(0.5)
+    P.upload(p)
+    V.upload(v)
+    ens.steps(5)
+    gpu.sync()
+    got = P.download(), V.download()
+    ens.close()
+    ref = _run_in_calls(gpu, p, v, n, fc, mc, [5])
+    assert np.array_equal(got[0], ref[0])
+    assert np.array_equal(got[1], ref[1])
