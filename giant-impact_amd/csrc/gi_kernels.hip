@@ -16,7 +16,10 @@
 //      diameter) built by counting sort (k_grid_*).
 //   3. k_finish   -- per particle: its all-pairs sum plus, for every pair
 //      with r < 1 found through the grid, the reference's exact piecewise
-//      contact force minus what pass 1 added for that pair.
+//      contact force minus what pass 1 added for that pair.  When every
+//      simulation takes the mutual pass, the same correction is computed by
+//      k_contact_corr beside pass 1 (it needs only the grid) and added by
+//      k_mutual_sum.
 //
 // Only pairs with r < 1 can differ from gravity (impactAddTime.cu:300,320,349),
 // so the sum equals the reference's force up to float rounding, and the
