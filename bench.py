@@ -370,6 +370,8 @@ def main() -> int:
     ap.add_argument("--ensemble-steps", type=int, default=5000,
                     help="batched steps of the ensemble leg (FP64 E / L every 100)")
     ap.add_argument("--only-ensemble", action="store_true", help="run the ensemble leg alone (profiling)")
+    ap.add_argument("--ensemble-trace", type=int, default=0, metavar="K",
+                    help="profiling: the ensemble leg's batched engine alone, 2 untimed steps then K timed")
     ap.add_argument("--only-small", action="store_true",
                     help="run the N=2,048 impact alone through the drop-in calls (profiling the small-N pass)")
     ap.add_argument("--serve-requests", type=int, default=24,
@@ -408,6 +410,23 @@ def main() -> int:
 
     if args.only_small:
         print(json.dumps({"small": small_leg(gi, 200)}), flush=True)
+        return 0
+
+    if args.ensemble_trace > 0:
+        # the batched engine on the ensemble leg's simulations, nothing else
+        pos, vel, n, fc, mc, _ = gi.make_ensemble(gi.ensemble_configs(args.ensemble_sims))
+        ntot = int(n.sum())
+        P, V = gi.DeviceArray(ntot, pos), gi.DeviceArray(ntot, vel)
+        ens = gi.Ensemble(P, V, n, fc, mc)
+        ens.steps(2)
+        gi.sync()
+        t0 = time.perf_counter()
+        ens.steps(args.ensemble_trace)
+        gi.sync()
+        dt = time.perf_counter() - t0
+        ens.close()
+        log(f"ensemble trace: {args.ensemble_trace} batched steps of {len(n)} simulations (sum N = {ntot}) "
+            f"in {dt * 1e3:.2f} ms ({dt / args.ensemble_trace * 1e3:.3f} ms/step)")
         return 0
 
     if args.profile_probe:
