@@ -114,6 +114,7 @@ struct Plan {
     // every simulation takes the mutual pass: the contact corrections run
     // beside it (k_contact_corr) and k_mutual_sum writes the force
     bool split_finish = false;
+    bool overlap_allowed = true;  // gi_set_contact_overlap when planned
     // direct pass (tiling.variant == 3): one workgroup per 64 / direct_lanes
     // i-particles
     bool direct = false;
@@ -369,6 +370,8 @@ static int plan_mutual(std::vector<SimDesc>& sims, const Tiling& tl, bool batche
 // and tasks for the rest, the contact-grid bucket ranges and the moment
 // blocks.  batched: every simulation uses the fixed tiling of the batched
 // engine, so its sums are the same alone and in any batch.
+static std::atomic<int> g_contact_overlap{1};  // see gi_set_contact_overlap
+
 static int build_plan(std::vector<SimDesc> sims, const Tiling& tl, bool batched, Plan* out) {
     Plan p;
     p.caller_sims = sims;
@@ -465,7 +468,8 @@ static int build_plan(std::vector<SimDesc> sims, const Tiling& tl, bool batched,
         }
         p.sim_first_block.push_back((int)p.block_sim.size());
     }
-    p.split_finish = p.mutual && p.tasks.empty() && !p.direct;
+    p.overlap_allowed = g_contact_overlap.load() != 0;
+    p.split_finish = p.mutual && p.tasks.empty() && !p.direct && p.overlap_allowed;
     p.sims = std::move(sims);
     *out = std::move(p);
     return 0;
@@ -490,7 +494,9 @@ static int ensure_plan(Workspace* ws, const std::vector<SimDesc>& sims, hipStrea
         const BatchedTiling bt = batched_tiling();
         tl = Tiling{bt.R, bt.variant, bt.chunk_len, 0, 0};
     }
-    if (ws->plan_valid && same_tiling(ws->plan.tiling, tl) && same_sims(ws->plan.caller_sims, sims)) return 0;
+    if (ws->plan_valid && same_tiling(ws->plan.tiling, tl) && same_sims(ws->plan.caller_sims, sims) &&
+        ws->plan.overlap_allowed == (g_contact_overlap.load() != 0))
+        return 0;
     Plan p;
     int rc = build_plan(sims, tl, ws->batched, &p);
     if (rc) return rc;
@@ -1148,6 +1154,11 @@ int gi_set_batched_tiling(int r, long long chunk_len, int variant) {
         return set_error(GI_E_INVALID_ARG,
                          "gi_set_batched_tiling: r in {1,2,4,8}, chunk_len >= 64, variant in {0,1,2}");
     batched_override() = BatchedTiling{(chunk_len + 63) / 64 * 64, r, variant};
+    return 0;
+}
+
+int gi_set_contact_overlap(int enable) {
+    g_contact_overlap.store(enable ? 1 : 0);
     return 0;
 }
 

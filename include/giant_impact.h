@@ -328,6 +328,16 @@ int gi_set_batched_tiling(int r, long long chunk_len, int variant);
  * No reference counterpart (the reference launches one N-thread grid). */
 int gi_set_dispatch_limit(long long max_tasks);
 
+/* Contact overlap (enable = 1, the default): when every simulation of a
+ * force evaluation takes the mutual all-pairs pass, its contact corrections
+ * are computed beside that pass on an internal stream and added with the
+ * pair sums (and a batched step also fuses its move there); with 0 they are
+ * added by a finishing kernel after the pass, and a batched step moves in a
+ * kernel of its own.  The two forms share their arithmetic: forces and
+ * states are bitwise the same either way.  Process-wide; a plan built under
+ * the other setting is rebuilt on its next use.  No reference counterpart. */
+int gi_set_contact_overlap(int enable);
+
 /* ---- kernel timing (HIP events around the all-pairs force kernel) ------- */
 
 /* Enable/disable per-launch timing (HIP events on the launching stream) of the

@@ -249,3 +249,58 @@ def test_large_simulation_mutual_pass_against_fp64(gpu, oracle):
         assert_force_parity(met)
     tot = F[:, :3].astype(np.float64).sum(0)
     assert np.abs(tot).max() <= 1e-5 * np.abs(F[:, :3]).astype(np.float64).sum()
+
+
+@pytest.fixture
+def contact_overlap(gpu):
+    def set_overlap(on):
+        gpu.call("gi_set_contact_overlap", 1 if on else 0)
+
+    yield set_overlap
+    gpu.call("gi_set_contact_overlap", 1)
+
+
+@pytest.mark.parametrize("R,L", [(4, 256), (8, 1024)])
+def test_contact_overlap_is_bitwise_the_same(gpu, tiling, contact_overlap, R, L):
+    """The contact corrections computed beside the all-pairs pass and added
+    with the sums, against the same corrections added after the join by
+    k_finish: one device function, one rounding sequence."""
+    pos, vel, fc, mc, _ = gpu.make_impact(4000, 1200, seed=31, angle_deg=40.0, speed_vesc=1.1,
+                                          gap_in_target_radii=0.2)
+    n = len(pos)
+    tiling(R, 2, L)
+    # a state with contacts: step the bodies into each other first
+    P, V, F = gpu.DeviceArray(n, pos), gpu.DeviceArray(n, vel), gpu.DeviceArray(n, f4(n))
+    gpu.call("gi_impact_steps", P.ptr, V.ptr, F.ptr, n, C.byref(fc), C.byref(mc), 300, 0, None, None, None)
+    gpu.sync()
+    p, v = P.download(), V.download()
+    print(f"n={n} R={R} L={L}", flush=True)
+    out = {}
+    for on in (True, False):
+        contact_overlap(on)
+        out[on] = _force(gpu, p, v, fc)
+        assert gpu.plan_info()["kernel"] == "k_mutual"
+    from scipy.spatial import cKDTree
+
+    assert len(cKDTree(p[:, :3].astype(np.float64)).query_pairs(1.0)) > n // 2  # contacts present
+    assert np.array_equal(out[True], out[False])
+
+
+def test_contact_overlap_batched_step_is_bitwise_the_same(gpu, batched_mutual, contact_overlap):
+    """The batched step with the corrections beside the pair pass and the move
+    fused into the sums, against the finishing kernel and a separate move."""
+    cfg = gpu.ensemble_configs(4, seed=12, n_range=(900, 3500))
+    pos, vel, n, fc, mc, _ = gpu.make_ensemble(cfg)
+    print("n =", list(n), flush=True)
+    out = {}
+    for on in (True, False):
+        contact_overlap(on)
+        P, V = gpu.DeviceArray(len(pos), pos), gpu.DeviceArray(len(pos), vel)
+        ens = gpu.Ensemble(P, V, n, fc, mc)
+        ens.steps(5)
+        ens.steps(20)
+        gpu.sync()
+        out[on] = (P.download(), V.download())
+        ens.close()
+    assert np.array_equal(out[True][0], out[False][0])
+    assert np.array_equal(out[True][1], out[False][1])
