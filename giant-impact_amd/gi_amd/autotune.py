@@ -33,7 +33,8 @@ CLASSES = [
     (3072, 1536, 512),
     (6144, 3072, 1024),
     (12288, 6144, 2048),
-    (24576, 12288, 4096),
+    (16384, 10752, 3584),
+    (24576, 15360, 5120),
     (49152, 24576, 8192),
     (98304, 49152, 16384),
     (393216, 196608, 65536),
