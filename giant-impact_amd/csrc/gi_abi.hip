@@ -744,6 +744,32 @@ static int launch_direct(DevCtx* ctx, Workspace* ws, const float4* pos, const fl
     return 0;
 }
 
+// The contact walk for plans of up to this many particles uses nine lanes per
+// particle: their all-pairs pass is too short to hide a one-lane walk.
+constexpr int kContactRowLanesMaxN = 49152;
+
+template <int LANES>
+static void launch_contact_lanes(const Workspace* ws, float4* force, hipStream_t st) {
+    const Plan& p = ws->plan;
+    const dim3 grid(contact_blocks<LANES>(p.ntot)), block(64 * kContactWaves);
+    if (force)
+        hipLaunchKernelGGL(k_finish<LANES>, grid, block, 0, st, force, p.ntot, ws->sims.p, (int)p.sims.size(),
+                           ws->partial.p, p.mutual ? ws->gsum.p : nullptr, ws->starts.p, ws->sorted.p, ws->spos.p,
+                           ws->svel.p, p.tiling.variant);
+    else
+        hipLaunchKernelGGL(k_contact_corr<LANES>, grid, block, 0, st, p.ntot, ws->sims.p, (int)p.sims.size(),
+                           ws->starts.p, ws->sorted.p, ws->spos.p, ws->svel.p, p.tiling.variant, ws->corr.p);
+}
+
+// With force: k_finish (sums + corrections into force); without: the
+// corrections alone into ws->corr (k_contact_corr).
+static void launch_contact(const Workspace* ws, float4* force, hipStream_t st) {
+    if (ws->plan.ntot <= kContactRowLanesMaxN)
+        launch_contact_lanes<kCellRows>(ws, force, st);
+    else
+        launch_contact_lanes<1>(ws, force, st);
+}
+
 // Full force evaluation for the current plan: grid, all pairs, finish (or
 // the direct pass alone).  With move_pos / move_vel (the arrays pos / vel
 // point to) and a plan with split_finish, the batched move (k_move) is fused
@@ -778,9 +804,7 @@ static int run_force(DevCtx* ctx, Workspace* ws, const float4* pos, const float4
             GI_HIP(hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
             if ((r = launch_grid(ws, pos, vel, ctx->side))) return r;
             if (p.split_finish) {
-                hipLaunchKernelGGL(k_contact_corr, dim3(nblk(p.ntot, 256)), dim3(256), 0, ctx->side, p.ntot,
-                                   ws->sims.p, (int)p.sims.size(), ws->starts.p, ws->sorted.p, ws->spos.p,
-                                   ws->svel.p, p.tiling.variant, ws->corr.p);
+                launch_contact(ws, nullptr, ctx->side);
                 GI_HIP(hipGetLastError());
             }
             GI_HIP(hipEventRecord(ctx->ev_join, ctx->side));
@@ -795,9 +819,7 @@ static int run_force(DevCtx* ctx, Workspace* ws, const float4* pos, const float4
         } else {
             if ((rc = launch_gravity(ctx, ws, pos, false, st, timed, nullptr, nullptr, fork))) return rc;
             GI_HIP(hipStreamWaitEvent(st, ctx->ev_join, 0));
-            hipLaunchKernelGGL(k_finish, dim3(nblk(p.ntot, 256)), dim3(256), 0, st, force, p.ntot, ws->sims.p,
-                               (int)p.sims.size(), ws->partial.p, p.mutual ? ws->gsum.p : nullptr, ws->starts.p,
-                               ws->sorted.p, ws->spos.p, ws->svel.p, p.tiling.variant);
+            launch_contact(ws, force, st);
             GI_HIP(hipGetLastError());
         }
     }

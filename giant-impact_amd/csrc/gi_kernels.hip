@@ -574,89 +574,160 @@ __global__ void k_grid_gather(int ntot, const int* __restrict__ sorted, const fl
 // Pass 3: chunk sums + exact contact correction
 // ---------------------------------------------------------------------------
 
-// The contact correction of particle i (at bucket position t of simulation
-// sd): for every pair with r < 1 found through the grid, the reference's
-// exact contact term minus what the all-pairs pass added for that pair.
-// Contact terms are rounded exactly as the reference rounds them (float
-// products of a float magnitude); only their sum is carried in FP64, so
-// cancellation between large contact forces adds no summation error.  Cells
-// are visited in a fixed order and a bucket lists its particles by index, so
-// the order of the terms does not depend on the rest of the launch.  A
-// simulation's buckets hold only its own particles, and i itself sits at
-// position t, so a candidate's index is loaded only for a pair in contact;
-// the three bucket ranges of a row of cells are loaded together.
-__device__ __forceinline__ void contact_correction(const SimDesc& sd, int t, int i,
-                                                   const int* __restrict__ starts,
-                                                   const int* __restrict__ sorted,
-                                                   const float4* __restrict__ spos,
-                                                   const float4* __restrict__ svel, int variant, double& ex,
-                                                   double& ey, double& ez) {
+// The contact correction of a particle, for every pair with r < 1 found
+// through the grid: the reference's exact contact term minus what the
+// all-pairs pass added for that pair.  Contact terms are rounded exactly as
+// the reference rounds them (float products of a float magnitude); only
+// their sum is carried in FP64, so cancellation between large contact forces
+// adds no summation error.
+//
+// The 27-cell walk is a chain of dependent loads (bucket bounds, then each
+// candidate).  It is cut into the nine rows of three cells (oz, oy) =
+// (r / 3 - 1, r % 3 - 1), r = 0..8; each row is summed from zero (cells in x
+// order, candidates in bucket order) and the nine row sums are added in row
+// order.  LANES = 9 walks the rows of a particle on nine lanes at once (seven
+// particles per wavefront, the row sums added through LDS): a chain a ninth
+// as long, for plans too small to hide the walk behind the all-pairs pass.
+// LANES = 1 walks them in turn on one lane.  The arithmetic is the same, so
+// the two forms give bitwise the same corrections.  A simulation's buckets
+// hold only its own particles, and the particle itself sits at position t,
+// so a candidate's index is loaded only for a pair in contact.
+constexpr int kCellRows = 9;
+constexpr int kContactWaves = 4;  // wavefronts per workgroup
+
+__device__ __forceinline__ void contact_row(const SimDesc& sd, int t, int i, int row,
+                                            const int* __restrict__ starts, const int* __restrict__ sorted,
+                                            const float4* __restrict__ spos, const float4* __restrict__ svel,
+                                            int variant, double& ex, double& ey, double& ez) {
     const float4 pi = spos[t];
     const float4 vi = svel[t];
     const int ccx = cell_coord(pi.x), ccy = cell_coord(pi.y), ccz = cell_coord(pi.z);
     const bool ife = i < sd.fe_end;
-    ex = 0.0;
-    ey = 0.0;
-    ez = 0.0;
-    for (int oz = -1; oz <= 1; ++oz)
-        for (int oy = -1; oy <= 1; ++oy) {
-            const int qy = ccy + oy, qz = ccz + oz;
-            int b0[3], b1[3];
+    const int qy = ccy + row % 3 - 1, qz = ccz + row / 3 - 1;
+    int b0[3], b1[3];
 #pragma unroll
-            for (int ox = 0; ox < 3; ++ox) {
-                const unsigned h = cell_bucket(sd.hbase, sd.hmask, ccx + ox - 1, qy, qz);
-                b0[ox] = starts[h];
-                b1[ox] = starts[h + 1];
-            }
+    for (int ox = 0; ox < 3; ++ox) {
+        const unsigned h = cell_bucket(sd.hbase, sd.hmask, ccx + ox - 1, qy, qz);
+        b0[ox] = starts[h];
+        b1[ox] = starts[h + 1];
+    }
 #pragma unroll
-            for (int ox = 0; ox < 3; ++ox) {
-                const int qx = ccx + ox - 1;
-                for (int k = b0[ox]; k < b1[ox]; ++k) {
-                    const float4 pj = spos[k];
-                    if (k == t || cell_coord(pj.x) != qx || cell_coord(pj.y) != qy || cell_coord(pj.z) != qz)
-                        continue;  // i itself, or another cell sharing the bucket
-                    const float dpx = pj.x - pi.x;
-                    const float dpy = pj.y - pi.y;
-                    const float dpz = pj.z - pi.z;
-                    const float r2 = dpx * dpx + dpy * dpy + dpz * dpz;
-                    const float r = sqrtf(r2);
-                    if (1.0f <= r) continue;
-                    const float invr = 1.0f / r;
-                    const int j = sorted[k];
-                    const int test = (int)ife + (int)(j < sd.fe_end);
-                    const float4 vj = svel[k];
-                    const float dvx = vj.x - vi.x, dvy = vj.y - vi.y, dvz = vj.z - vi.z;
-                    const float dot = dpx * dvx + dpy * dvy + dpz * dvz;
-                    const float fm = contact_pair_mag(sd.law, test, r, r2, dot);
-                    // What pass 1 added for this pair.
-                    const float g = gravity_product(sd.law, test);
-                    const float sf = g * inv_r3_variant(variant, fast_r2(dpx, dpy, dpz));
-                    ex += (double)(fm * dpx * invr) - (double)(sf * dpx);
-                    ey += (double)(fm * dpy * invr) - (double)(sf * dpy);
-                    ez += (double)(fm * dpz * invr) - (double)(sf * dpz);
-                }
-            }
+    for (int ox = 0; ox < 3; ++ox) {
+        const int qx = ccx + ox - 1;
+        for (int k = b0[ox]; k < b1[ox]; ++k) {
+            const float4 pj = spos[k];
+            if (k == t || cell_coord(pj.x) != qx || cell_coord(pj.y) != qy || cell_coord(pj.z) != qz)
+                continue;  // i itself, or another cell sharing the bucket
+            const float dpx = pj.x - pi.x;
+            const float dpy = pj.y - pi.y;
+            const float dpz = pj.z - pi.z;
+            const float r2 = dpx * dpx + dpy * dpy + dpz * dpz;
+            const float r = sqrtf(r2);
+            if (1.0f <= r) continue;
+            const float invr = 1.0f / r;
+            const int j = sorted[k];
+            const int test = (int)ife + (int)(j < sd.fe_end);
+            const float4 vj = svel[k];
+            const float dvx = vj.x - vi.x, dvy = vj.y - vi.y, dvz = vj.z - vi.z;
+            const float dot = dpx * dvx + dpy * dvy + dpz * dvz;
+            const float fm = contact_pair_mag(sd.law, test, r, r2, dot);
+            // What pass 1 added for this pair.
+            const float g = gravity_product(sd.law, test);
+            const float sf = g * inv_r3_variant(variant, fast_r2(dpx, dpy, dpz));
+            ex += (double)(fm * dpx * invr) - (double)(sf * dpx);
+            ey += (double)(fm * dpy * invr) - (double)(sf * dpy);
+            ez += (double)(fm * dpz * invr) - (double)(sf * dpz);
         }
+    }
 }
 
-__global__ __launch_bounds__(256) void k_finish(float4* __restrict__ force, int ntot,
-                                                const SimDesc* __restrict__ sims, int nsims,
-                                                const float4* __restrict__ partial,
-                                                const float4* __restrict__ gsum,
-                                                const int* __restrict__ starts,
-                                                const int* __restrict__ sorted,
-                                                const float4* __restrict__ spos,
-                                                const float4* __restrict__ svel, int variant) {
-    // Threads walk the particles in bucket (Morton) order: the lanes of a
-    // wavefront look up the same and adjacent cells.
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= ntot) return;
-    const int i = sorted[t];
-    const int s = find_sim(sims, nsims, i);
-    const SimDesc& sd = sims[s];
-    if (!sd.active) return;
-    const int n = sd.end - sd.start;
+// Particles per wavefront of the contact walk.
+template <int LANES>
+__host__ __device__ constexpr int contact_per_wave() {
+    return LANES == 1 ? 64 : 64 / LANES;
+}
 
+// Workgroups of k_finish / k_contact_corr for ntot particles.
+template <int LANES>
+__host__ __device__ constexpr int contact_blocks(int ntot) {
+    return (ntot + kContactWaves * contact_per_wave<LANES>() - 1) / (kContactWaves * contact_per_wave<LANES>());
+}
+
+// The lane's particle (bucket position t, index i, simulation s; t = -1 for
+// none) and, on the lane that owns the result (`lead`), its correction.
+struct ContactLane {
+    int t, i, s;
+    bool lead;
+    double ex, ey, ez;
+};
+
+template <int LANES>
+__device__ __forceinline__ ContactLane contact_correction(int ntot, const SimDesc* __restrict__ sims, int nsims,
+                                                          const int* __restrict__ starts,
+                                                          const int* __restrict__ sorted,
+                                                          const float4* __restrict__ spos,
+                                                          const float4* __restrict__ svel, int variant) {
+    static_assert(LANES == 1 || LANES == kCellRows, "one lane per particle or one per cell row");
+    const int lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+    const int slot = lane / LANES;  // LANES = 9: 0..6, and 7 for the idle lane 63
+    const int row = LANES == 1 ? 0 : lane % LANES;
+    ContactLane c;
+    c.t = slot < contact_per_wave<LANES>()
+              ? (blockIdx.x * kContactWaves + w) * contact_per_wave<LANES>() + slot
+              : -1;
+    if (c.t >= ntot) c.t = -1;
+    c.i = c.t >= 0 ? sorted[c.t] : 0;
+    c.s = c.t >= 0 ? find_sim(sims, nsims, c.i) : 0;
+    const bool live = c.t >= 0 && sims[c.s].active;
+    c.ex = c.ey = c.ez = 0.0;
+    if constexpr (LANES == 1) {
+        c.lead = live;
+        if (live)
+            for (int r = 0; r < kCellRows; ++r) {
+                double rx = 0.0, ry = 0.0, rz = 0.0;
+                contact_row(sims[c.s], c.t, c.i, r, starts, sorted, spos, svel, variant, rx, ry, rz);
+                c.ex += rx;
+                c.ey += ry;
+                c.ez += rz;
+            }
+    } else {
+        __shared__ double part[kContactWaves][3][64];
+        double rx = 0.0, ry = 0.0, rz = 0.0;
+        if (live) contact_row(sims[c.s], c.t, c.i, row, starts, sorted, spos, svel, variant, rx, ry, rz);
+        part[w][0][lane] = rx;
+        part[w][1][lane] = ry;
+        part[w][2][lane] = rz;
+        __syncthreads();
+        c.lead = live && row == 0;
+        if (c.lead) {
+#pragma unroll
+            for (int r = 0; r < kCellRows; ++r) {
+                c.ex += part[w][0][lane + r];
+                c.ey += part[w][1][lane + r];
+                c.ez += part[w][2][lane + r];
+            }
+        }
+    }
+    return c;
+}
+
+// Threads walk the particles in bucket (Morton) order: the lanes of a
+// wavefront look up the same and adjacent cells.
+template <int LANES>
+__global__ __launch_bounds__(64 * kContactWaves) void k_finish(float4* __restrict__ force, int ntot,
+                                                               const SimDesc* __restrict__ sims, int nsims,
+                                                               const float4* __restrict__ partial,
+                                                               const float4* __restrict__ gsum,
+                                                               const int* __restrict__ starts,
+                                                               const int* __restrict__ sorted,
+                                                               const float4* __restrict__ spos,
+                                                               const float4* __restrict__ svel, int variant) {
+    const ContactLane c = contact_correction<LANES>(ntot, sims, nsims, starts, sorted, spos, svel, variant);
+    if (!c.lead) return;
+    const SimDesc& sd = sims[c.s];
+    const int i = c.i;
+    const int n = sd.end - sd.start;
     float fx = 0.0f, fy = 0.0f, fz = 0.0f;
     if (gsum && sd.gn > 0) {
         // mutual pass: already summed per particle
@@ -666,8 +737,8 @@ __global__ __launch_bounds__(256) void k_finish(float4* __restrict__ force, int 
         fz = gs.z;
     } else {
         float kx = 0.0f, ky = 0.0f, kz = 0.0f;
-        for (int c = 0; c < sd.nchunk; ++c) {
-            float4 p = partial[sd.partial_base + c * n + (i - sd.start)];
+        for (int ch = 0; ch < sd.nchunk; ++ch) {
+            float4 p = partial[sd.partial_base + ch * n + (i - sd.start)];
             kahan_add(fx, kx, p.x);
             kahan_add(fy, ky, p.y);
             kahan_add(fz, kz, p.z);
@@ -676,33 +747,27 @@ __global__ __launch_bounds__(256) void k_finish(float4* __restrict__ force, int 
         fy -= ky;
         fz -= kz;
     }
-    double ex, ey, ez;
-    contact_correction(sd, t, i, starts, sorted, spos, svel, variant, ex, ey, ez);
-    force[i].x = (float)((double)fx + ex);
-    force[i].y = (float)((double)fy + ey);
-    force[i].z = (float)((double)fz + ez);
+    force[i].x = (float)((double)fx + c.ex);
+    force[i].y = (float)((double)fy + c.ey);
+    force[i].z = (float)((double)fz + c.ez);
 }
 
 // The contact corrections alone, into corr (three planes of ntot doubles,
 // by particle index), for plans whose simulations all take the mutual pass:
 // it needs only the grid, so it runs beside the all-pairs pass, and
 // k_mutual_sum adds it to the sums -- the same arithmetic as k_finish.
-__global__ __launch_bounds__(256) void k_contact_corr(int ntot, const SimDesc* __restrict__ sims, int nsims,
-                                                      const int* __restrict__ starts,
-                                                      const int* __restrict__ sorted,
-                                                      const float4* __restrict__ spos,
-                                                      const float4* __restrict__ svel, int variant,
-                                                      double* __restrict__ corr) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= ntot) return;
-    const int i = sorted[t];
-    const SimDesc& sd = sims[find_sim(sims, nsims, i)];
-    if (!sd.active) return;
-    double ex, ey, ez;
-    contact_correction(sd, t, i, starts, sorted, spos, svel, variant, ex, ey, ez);
-    corr[i] = ex;
-    corr[ntot + i] = ey;
-    corr[2 * ntot + i] = ez;
+template <int LANES>
+__global__ __launch_bounds__(64 * kContactWaves) void k_contact_corr(int ntot, const SimDesc* __restrict__ sims,
+                                                                     int nsims, const int* __restrict__ starts,
+                                                                     const int* __restrict__ sorted,
+                                                                     const float4* __restrict__ spos,
+                                                                     const float4* __restrict__ svel, int variant,
+                                                                     double* __restrict__ corr) {
+    const ContactLane c = contact_correction<LANES>(ntot, sims, nsims, starts, sorted, spos, svel, variant);
+    if (!c.lead) return;
+    corr[c.i] = c.ex;
+    corr[ntot + c.i] = c.ey;
+    corr[2 * ntot + c.i] = c.ez;
 }
 
 // ---------------------------------------------------------------------------

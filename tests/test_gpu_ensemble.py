@@ -165,3 +165,26 @@ def test_positions_changed_between_calls_are_used(gpu):
     ref = _run_in_calls(gpu, p, v, n, fc, mc, [5])
     assert np.array_equal(got[0], ref[0])
     assert np.array_equal(got[1], ref[1])
+
+
+def test_contact_walk_forms_are_bitwise_the_same(gpu):
+    """Plans of up to 49,152 particles walk each particle's contact cells on
+    nine lanes, larger ones on one lane; both add the same nine cell-row sums
+    in the same order.  Here each simulation alone takes the nine-lane walk
+    and the mix of all five (about 75,000 particles) the one-lane walk."""
+    cfg = gpu.ensemble_configs(5, seed=21, n_range=(14000, 16000))
+    pos, vel, n, fc, mc, _ = gpu.make_ensemble(cfg)
+    assert n.sum() > 49152 and n.max() <= 49152
+    print("n =", list(n), flush=True)
+    P, V = gpu.DeviceArray(len(pos), pos), gpu.DeviceArray(len(pos), vel)
+    ens = gpu.Ensemble(P, V, n, fc, mc)
+    ens.steps(4)
+    gpu.sync()
+    p_all, v_all = P.download(), V.download()
+    off = ens.offsets
+    ens.close()
+    for s in range(len(n)):
+        a, b = off[s], off[s + 1]
+        p1, v1 = _ensemble_of_one(gpu, pos[a:b], vel[a:b], fc[s], mc[s], 4)
+        assert np.array_equal(p_all[a:b], p1), s
+        assert np.array_equal(v_all[a:b], v1), s
