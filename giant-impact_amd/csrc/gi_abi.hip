@@ -632,7 +632,7 @@ static void launch_mutual_kernel(int R, hipStream_t st, const Workspace* ws, int
 static int launch_gravity(DevCtx* ctx, Workspace* ws, const float4* pos, bool pot, hipStream_t st,
                           bool timed, float4* force = nullptr, hipEvent_t join = nullptr,
                           const std::function<int()>& after_pad = nullptr, float4* mpos = nullptr,
-                          float4* mvel = nullptr) {
+                          float4* mvel = nullptr, const SumMove& mv = SumMove{0.0f, 0.0f, 0.0f, 1}) {
     const Plan& p = ws->plan;
     const int nplain = (int)p.tasks.size();
     const int nmut = (int)p.mtasks.size();
@@ -687,7 +687,7 @@ static int launch_gravity(DevCtx* ctx, Workspace* ws, const float4* pos, bool po
         hipLaunchKernelGGL(k_mutual_sum, dim3((unsigned)p.sblocks.size()), dim3(64 * kSumSplit), 0, st,
                            ws->sblocks.p, ws->sims.p, ws->rows.p, 64 * p.tiling.R, ws->iside.p, ws->jside.p,
                            ws->gsum.p, pot ? 1 : 0, force ? (const double*)ws->corr.p : nullptr, p.ntot, force, mpos,
-                           mvel);
+                           mvel, mv);
         GI_HIP(hipGetLastError());
     }
     if (timed) {
@@ -772,11 +772,12 @@ static void launch_contact(const Workspace* ws, float4* force, hipStream_t st) {
 
 // Full force evaluation for the current plan: grid, all pairs, finish (or
 // the direct pass alone).  With move_pos / move_vel (the arrays pos / vel
-// point to) and a plan with split_finish, the batched move (k_move) is fused
-// into the last kernel and *moved is set; otherwise the caller moves.
+// point to) and a plan with split_finish, the move is fused into the last
+// kernel and *moved is set; otherwise the caller moves.  The move takes each
+// simulation's own constants (the batched engine), or mv's when given.
 static int run_force(DevCtx* ctx, Workspace* ws, const float4* pos, const float4* vel, float4* force,
                      hipStream_t st, bool timed, float4* move_pos = nullptr, float4* move_vel = nullptr,
-                     bool* moved = nullptr) {
+                     bool* moved = nullptr, const SumMove* mv = nullptr) {
     const Plan& p = ws->plan;
     if (p.ntot == 0) return 0;
     TimingPair whole;
@@ -813,7 +814,8 @@ static int run_force(DevCtx* ctx, Workspace* ws, const float4* pos, const float4
         if (p.split_finish) {
             const bool fuse = move_pos && move_vel && moved;
             if ((rc = launch_gravity(ctx, ws, pos, false, st, timed, force, ctx->ev_join, fork,
-                                     fuse ? move_pos : nullptr, fuse ? move_vel : nullptr)))
+                                     fuse ? move_pos : nullptr, fuse ? move_vel : nullptr,
+                                     mv ? *mv : SumMove{0.0f, 0.0f, 0.0f, 1})))
                 return rc;
             if (fuse) *moved = true;
         } else {
@@ -1477,12 +1479,17 @@ int gi_impact_steps(gi_float4* pos, gi_float4* vel, gi_float4* force, int n, con
     key_append(key, n);
     key_append(key, *fc);
     key_append(key, m);
+    // The move is fused into the force's last kernel when the plan allows
+    // (k_move_collision's arithmetic, with these constants; nfe = NFe).
+    const SumMove smv{m.Dt, m.DtOverMassFe, m.DtOverMassSi, 0};
     auto body = [&](hipStream_t cs, int steps) -> int {
         for (int k = 0; k < steps; ++k) {
-            int r = run_force(ctx, &ctx->ws, P, V, F, cs, false);
+            bool moved = false;
+            int r = run_force(ctx, &ctx->ws, P, V, F, cs, false, P, V, &moved, &smv);
             if (r) return r;
-            hipLaunchKernelGGL(k_move_collision, dim3(nblk(n, 256)), dim3(256), 0, cs, P, V, (const float4*)F, n,
-                               m.Dt, m.DtOverMassFe, m.DtOverMassSi, m.NFe);
+            if (!moved)
+                hipLaunchKernelGGL(k_move_collision, dim3(nblk(n, 256)), dim3(256), 0, cs, P, V, (const float4*)F,
+                                   n, m.Dt, m.DtOverMassFe, m.DtOverMassSi, m.NFe);
             GI_HIP(hipGetLastError());
         }
         return 0;

@@ -358,9 +358,17 @@ __global__ void k_mutual_pad(const float4* __restrict__ pos, int gtot, const Sim
 // times the loads in flight of one thread per particle.
 // With corr (the contact corrections of k_contact_corr), the sum rounded to
 // float plus the correction goes to force.xyz -- k_finish's arithmetic --
-// and, with mpos / mvel, the particle then takes k_move's step; without
-// corr, the sum goes to gsum for k_finish / k_finish_potential.
+// and, with mpos / mvel, the particle then takes the move's step (k_move's
+// and k_move_collision's arithmetic) with the constants of mv, or with its
+// simulation's own when mv.per_sim; without corr, the sum goes to gsum for
+// k_finish / k_finish_potential.
 constexpr int kSumSplit = 4;
+
+// Move constants of a move fused into k_mutual_sum.
+struct SumMove {
+    float dt, dt_over_mass_fe, dt_over_mass_si;
+    int per_sim;  // 1: take them from each SimDesc (the batched engine)
+};
 
 __global__ __launch_bounds__(64 * kSumSplit) void k_mutual_sum(const SumBlock* __restrict__ blocks,
                                                               const SimDesc* __restrict__ sims,
@@ -371,7 +379,7 @@ __global__ __launch_bounds__(64 * kSumSplit) void k_mutual_sum(const SumBlock* _
                                                               const double* __restrict__ corr, int ntot,
                                                               float4* __restrict__ force,
                                                               float4* __restrict__ mpos,
-                                                              float4* __restrict__ mvel) {
+                                                              float4* __restrict__ mvel, SumMove mv) {
     __shared__ double red[3][kSumSplit][64];
     const int lane = threadIdx.x & 63;
     const int w = threadIdx.x >> 6;
@@ -428,16 +436,18 @@ __global__ __launch_bounds__(64 * kSumSplit) void k_mutual_sum(const SumBlock* _
             force[i].y = fy;
             force[i].z = fz;
             if (mpos) {
-                // k_move's arithmetic
-                const float temp = i < sd.fe_end ? sd.dt_over_mass_fe : sd.dt_over_mass_si;
+                const float dt = mv.per_sim ? sd.dt : mv.dt;
+                const float dtm_fe = mv.per_sim ? sd.dt_over_mass_fe : mv.dt_over_mass_fe;
+                const float dtm_si = mv.per_sim ? sd.dt_over_mass_si : mv.dt_over_mass_si;
+                const float temp = i < sd.fe_end ? dtm_fe : dtm_si;
                 float4 v = mvel[i];
                 float4 p = mpos[i];
                 v.x += fx * temp;
                 v.y += fy * temp;
                 v.z += fz * temp;
-                p.x += v.x * sd.dt;
-                p.y += v.y * sd.dt;
-                p.z += v.z * sd.dt;
+                p.x += v.x * dt;
+                p.y += v.y * dt;
+                p.z += v.z * dt;
                 mvel[i] = v;
                 mpos[i] = p;
             }

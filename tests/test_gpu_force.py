@@ -173,7 +173,17 @@ def test_large_n_sample_against_fp64(gpu, oracle):
     assert np.abs(tot).max() <= 1e-5 * np.abs(F[:, :3]).astype(np.float64).sum()
 
 
-def test_graph_stepper_equals_the_manual_loop(gpu):
+@pytest.fixture(params=["table", "mutual"])
+def stepper_tiling(gpu, request):
+    """The table's pass at N = 2,048 (the direct pass, move fused in the
+    eight-step graphs) or the three-pass form (move fused into its sums)."""
+    if request.param == "mutual":
+        gpu.call("gi_set_tiling", 4, 2, 256, 0, 64)
+    yield request.param
+    gpu.call("gi_set_tiling", 0, 0, 0, 0, 0)
+
+
+def test_graph_stepper_equals_the_manual_loop(gpu, stepper_tiling):
     pos, vel, fc, mc, _ = gpu.make_impact(1536, 512, seed=9)
     n = len(pos)
     P1, V1, F1 = _upload(gpu, pos, vel)
@@ -193,7 +203,7 @@ def test_graph_stepper_equals_the_manual_loop(gpu):
 
 
 @pytest.mark.parametrize("rate", [3, 8, 10])
-def test_graph_stepper_with_recenter_equals_the_manual_loop(gpu, rate):
+def test_graph_stepper_with_recenter_equals_the_manual_loop(gpu, stepper_tiling, rate):
     """25 steps with a recenter every `rate` steps: the one- and eight-step
     graphs and the recenter graph, against single calls, bitwise."""
     pos, vel, fc, mc, _ = gpu.make_impact(1536, 512, seed=13, angle_deg=30.0, speed_vesc=1.3)
@@ -202,7 +212,7 @@ def test_graph_stepper_with_recenter_equals_the_manual_loop(gpu, rate):
     tv = (C.c_double * 3)(0.01, 0.0, -0.02)
     mfe = float(np.float64(np.float32(mc.Dt)) / np.float64(np.float32(mc.DtOverMassFe)))
     msi = float(np.float64(np.float32(mc.Dt)) / np.float64(np.float32(mc.DtOverMassSi)))
-    print(f"rate={rate}", flush=True)
+    print(f"rate={rate} tiling={stepper_tiling}", flush=True)
     P1, V1, F1 = _upload(gpu, pos, vel)
     for s in range(1, 26):
         gpu.force_collision(P1, V1, F1, n, fc)
