@@ -219,8 +219,17 @@ __global__ __launch_bounds__(64 * kMutualWaves) void k_mutual(const float4* __re
         ax[r] = ay[r] = az[r] = 0.0f;
     }
 
-    if (t.diag) {
-        // The row against itself, one-sided (includes the self pair).
+    // The block's positions, twice, in wave-private LDS: q[m] holds
+    // j_((127 - m) mod 64), so step k reads q[63 - lane + k] =
+    // j_((lane - k) mod 64) -- the j whose accumulator lane `lane` holds
+    // after k rotations -- at the immediate offset 16 k.
+    __shared__ float4 lq[kMutualWaves][128];
+    float4* __restrict__ q = lq[wv];
+
+    if (R > 4 && t.diag) {
+        // The row against itself, one-sided (includes the self pair): at
+        // R = 8 this keeps the off-diagonal loop's register allocation (the
+        // mutual form below costs that loop 4%).
 #pragma unroll 4
         for (int j = t.j0; j < t.j1; ++j) {
             const float4 pj = P[j];
@@ -240,16 +249,74 @@ __global__ __launch_bounds__(64 * kMutualWaves) void k_mutual(const float4* __re
                 }
             }
         }
+    } else if (t.diag) {
+        // The row against itself, each pair once: its 64-blocks m holding a
+        // real particle against the row's sub-blocks r < m (both sides, as
+        // below), and against sub-block m itself over rotations 1..32 (lanes
+        // 0..31 only at 32): every pair of the block exactly once, the self
+        // pair never.  The j-side sums return to their lanes after 64
+        // rotations and belong to sub-block m's particles.
+        const int nb = (t.j1 - t.j0) >> 6;
+        for (int m = 0; m < nb; ++m) {
+            const float4 own = P[ib + m * 64 + lane];
+            wave_lds_sync();  // the previous block's reads are done
+            q[63 - lane] = own;
+            q[127 - lane] = own;
+            wave_lds_sync();
+            float xj = own.x, yj = own.y, zj = own.z;
+            const float4* __restrict__ qb = q + (63 - lane);
+            float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+#pragma unroll 4
+            for (int k = 0; k < 64; ++k) {
+                const float4 pn = qb[k + 1 < 64 ? k + 1 : 63];  // next step's j
+                const bool self_on = (k >= 1 && k < 32) || (k == 32 && lane < 32);
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    if (r > m || (r == m && !self_on)) continue;
+                    const float dx = xj - xi[r];
+                    const float dy = yj - yi[r];
+                    const float dz = zj - zi[r];
+                    if constexpr (!POT) {
+                        const float s = fast_inv_r3(fast_r2(dx, dy, dz));
+                        ax[r] = __builtin_fmaf(s, dx, ax[r]);
+                        ay[r] = __builtin_fmaf(s, dy, ay[r]);
+                        az[r] = __builtin_fmaf(s, dz, az[r]);
+                        fx = __builtin_fmaf(s, dx, fx);
+                        fy = __builtin_fmaf(s, dy, fy);
+                        fz = __builtin_fmaf(s, dz, fz);
+                    } else {
+                        const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+                        const float ri = (r2 > 0.0f && r2 < kGhostR2) ? __builtin_amdgcn_rsqf(r2) : 0.0f;
+                        ax[r] += ri;
+                        fx += ri;
+                    }
+                }
+                xj = pn.x;
+                yj = pn.y;
+                zj = pn.z;
+                fx = rotate_lanes(fx);
+                if constexpr (!POT) {
+                    fy = rotate_lanes(fy);
+                    fz = rotate_lanes(fz);
+                }
+            }
+            // The force on j is the negated sum; the potential is symmetric.
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+                if (r == m) {
+                    if constexpr (!POT) {
+                        ax[r] -= fx;
+                        ay[r] -= fy;
+                        az[r] -= fz;
+                    } else {
+                        ax[r] += fx;
+                    }
+                }
+        }
     } else {
         const int first = (t.row + 1) * B;  // first j of the row's slab
         const long long len = sd.gn - first;
         float* __restrict__ J = jside + sd.jbase + 3 * jside_row_offset(t.row, sd.gn, B);
-        // The block's positions, twice, in wave-private LDS: q[m] holds
-        // j_((127 - m) mod 64), so step k reads q[63 - lane + k] =
-        // j_((lane - k) mod 64) -- the j whose accumulator lane `lane` holds
-        // after k rotations -- at the immediate offset 16 k.
-        __shared__ float4 lq[kMutualWaves][128];
-        float4* __restrict__ q = lq[wv];
         float4 nx = P[t.j0 + lane];
         for (int J0 = t.j0; J0 < t.j1; J0 += 64) {
             float xj = nx.x, yj = nx.y, zj = nx.z;
