@@ -627,18 +627,22 @@ static void launch_mutual_kernel(int R, hipStream_t st, const Workspace* ws, int
 // the all-pairs kernel(s), kind 2 the whole pass.  With force (plans with
 // split_finish), st waits for `join` (the contact corrections in ws->corr)
 // and the sums go to force with the corrections added (and with mpos / mvel
-// the batched move is applied).  after_pad runs right after the padded copy
-// is enqueued (work forked from there does not compete with the copy).
+// the batched move is applied).  With fork, the event is recorded on st
+// right after the padded copy, and side_work -- the work that waits for it
+// on another stream -- is enqueued after the pair kernels, so that a caller
+// enqueueing call by call does not hold them back (work forked from there
+// does not compete with the copy).
 static int launch_gravity(DevCtx* ctx, Workspace* ws, const float4* pos, bool pot, hipStream_t st,
-                          bool timed, float4* force = nullptr, hipEvent_t join = nullptr,
-                          const std::function<int()>& after_pad = nullptr, float4* mpos = nullptr,
+                          bool timed, float4* force = nullptr, hipEvent_t join = nullptr, hipEvent_t fork = nullptr,
+                          const std::function<int()>& side_work = nullptr, float4* mpos = nullptr,
                           float4* mvel = nullptr, const SumMove& mv = SumMove{0.0f, 0.0f, 0.0f, 1}) {
     const Plan& p = ws->plan;
     const int nplain = (int)p.tasks.size();
     const int nmut = (int)p.mtasks.size();
     int rc;
     if (nplain == 0 && nmut == 0) {
-        if (after_pad && (rc = after_pad())) return rc;
+        if (fork) GI_HIP(hipEventRecord(fork, st));
+        if (side_work && (rc = side_work())) return rc;
         if (join) GI_HIP(hipStreamWaitEvent(st, join, 0));
         return 0;
     }
@@ -651,7 +655,7 @@ static int launch_gravity(DevCtx* ctx, Workspace* ws, const float4* pos, bool po
     if (nmut > 0)
         hipLaunchKernelGGL(k_mutual_pad, dim3(nblk(p.gtot, 256)), dim3(256), 0, st, pos, (int)p.gtot, ws->sims.p,
                            nsims, ws->gpos.p);
-    if (after_pad && (rc = after_pad())) return rc;
+    if (fork) GI_HIP(hipEventRecord(fork, st));
     if (timed) {
         tp = take_pair(ctx, 0);
         GI_HIP(hipEventRecord(tp.a, st));
@@ -682,6 +686,7 @@ static int launch_gravity(DevCtx* ctx, Workspace* ws, const float4* pos, bool po
         GI_HIP(hipEventRecord(tp.b, st));
         ctx->pending.push_back(tp);
     }
+    if (side_work && (rc = side_work())) return rc;
     if (join) GI_HIP(hipStreamWaitEvent(st, join, 0));
     if (nmut > 0 && !p.sblocks.empty()) {
         hipLaunchKernelGGL(k_mutual_sum, dim3((unsigned)p.sblocks.size()), dim3(64 * kSumSplit), 0, st,
@@ -799,9 +804,8 @@ static int run_force(DevCtx* ctx, Workspace* ws, const float4* pos, const float4
         if (!ctx->side) GI_HIP(hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking));
         if (!ctx->ev_fork) GI_HIP(hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));
         if (!ctx->ev_join) GI_HIP(hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming));
-        const std::function<int()> fork = [&]() -> int {
+        const std::function<int()> side_work = [&]() -> int {
             int r;
-            GI_HIP(hipEventRecord(ctx->ev_fork, st));
             GI_HIP(hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
             if ((r = launch_grid(ws, pos, vel, ctx->side))) return r;
             if (p.split_finish) {
@@ -813,13 +817,14 @@ static int run_force(DevCtx* ctx, Workspace* ws, const float4* pos, const float4
         };
         if (p.split_finish) {
             const bool fuse = move_pos && move_vel && moved;
-            if ((rc = launch_gravity(ctx, ws, pos, false, st, timed, force, ctx->ev_join, fork,
+            if ((rc = launch_gravity(ctx, ws, pos, false, st, timed, force, ctx->ev_join, ctx->ev_fork, side_work,
                                      fuse ? move_pos : nullptr, fuse ? move_vel : nullptr,
                                      mv ? *mv : SumMove{0.0f, 0.0f, 0.0f, 1})))
                 return rc;
             if (fuse) *moved = true;
         } else {
-            if ((rc = launch_gravity(ctx, ws, pos, false, st, timed, nullptr, nullptr, fork))) return rc;
+            if ((rc = launch_gravity(ctx, ws, pos, false, st, timed, nullptr, nullptr, ctx->ev_fork, side_work)))
+                return rc;
             GI_HIP(hipStreamWaitEvent(st, ctx->ev_join, 0));
             launch_contact(ws, force, st);
             GI_HIP(hipGetLastError());
