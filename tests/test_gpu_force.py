@@ -36,11 +36,27 @@ CASES = {
 }
 
 
+FORMS = {
+    # name: gi_set_tiling arguments (r, variant, chunk_len, target_tasks, min_chunk); r = 0 keeps the table
+    "table": (0, 0, 0, 0, 0),  # the direct pass at these sizes
+    "one-sided": (1, 0, 0, 8192, 512),  # k_gravity, then k_finish (nine-lane contact walk)
+    "mutual": (4, 2, 256, 0, 64),  # k_mutual beside k_contact_corr, k_mutual_sum
+}
+
+
+@pytest.fixture(params=sorted(FORMS))
+def force_form(gpu, request):
+    """The force in each of its forms: the table's pass and both three-pass forms."""
+    gpu.call("gi_set_tiling", *FORMS[request.param])
+    yield request.param
+    gpu.call("gi_set_tiling", 0, 0, 0, 0, 0)
+
+
 @pytest.mark.parametrize("case", sorted(CASES))
-def test_force_collision_parity(gpu, oracle, case):
+def test_force_collision_parity(gpu, oracle, case, force_form):
     nt, ni, seed, ang, spd, spin = CASES[case]
     pos, vel, fc, _, _ = gpu.make_impact(nt, ni, seed=seed, angle_deg=ang, speed_vesc=spd, target_spin=spin)
-    print(f"case={case} n={nt + ni}", flush=True)
+    print(f"case={case} n={nt + ni} form={force_form}", flush=True)
     F = _gpu_force(gpu, pos, vel, fc)
     F64, cond, near = oracle.force_collision_f64(pos, vel, fc)
     Fref = oracle.force_collision_f32(pos, vel, fc)
@@ -50,7 +66,7 @@ def test_force_collision_parity(gpu, oracle, case):
     assert np.all(F[:, 3] == 0.0)  # .w untouched (was zero)
 
 
-def test_force_collision_during_impact(gpu, oracle):
+def test_force_collision_during_impact(gpu, oracle, force_form):
     """After 400 steps the bodies are in contact: receding branches, deep
     overlaps and Fe-Si pairs all occur."""
     pos, vel, fc, mc, _ = gpu.make_impact(1536, 512, seed=20261202, angle_deg=45.0, speed_vesc=1.2,
