@@ -107,9 +107,10 @@ def _generator_bodies(gi, n1, n2, seed):
     return pos, vel, fs, ms
 
 
-def test_force_separate_parity(gpu, oracle):
+def test_force_separate_parity(gpu, oracle, force_form):
     pos, vel, fs, _ = _generator_bodies(gpu, 1200, 500, 21)
     n = len(pos)
+    print(f"n={n} form={force_form}", flush=True)
     P, V, F = _upload(gpu, pos, vel)
     gpu.force_separate(P, V, F, n, fs)
     gpu.sync()
