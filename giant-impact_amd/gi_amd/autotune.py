@@ -122,7 +122,7 @@ def main(argv=None) -> int:
     # at N = 262,144); --variants sweeps it again
     variants = [0, 1] if args.variants else [0]
     targets = [4096, 8192, 16384, 32768, 65536, 131072] if not args.quick else [32768]
-    mutual_Ls = [256, 512, 1024, 2048] if not args.quick else [512]
+    mutual_Ls = [192, 256, 384, 512, 1024, 2048] if not args.quick else [512]
     rows = []
     only = {int(x) for x in args.classes.split(",") if x.strip()}
     kept = {}
