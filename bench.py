@@ -414,8 +414,8 @@ def main() -> int:
 
     if args.gpus != 1:
         print(
-            "bench.py measures one GPU: this build runs on a single MI355X and has no multi-GPU path "
-            "(the sharding design is described in DESIGN.md). Use --gpus 1.",
+            "bench.py measures one GPU: this implementation runs on a single MI355X and has no "
+            "multi-GPU path. Use --gpus 1.",
             file=sys.stderr,
         )
         return 2
