@@ -11,7 +11,7 @@ reads on first use:
 
     n_max R variant chunk_len target_tasks min_chunk
 
-Usage (on the GPU box):  python -m gi_amd.autotune [--quick] [--out PATH]
+Usage (on an MI355X):  python -m gi_amd.autotune [--quick] [--out PATH]
 """
 from __future__ import annotations
 
