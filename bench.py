@@ -569,7 +569,14 @@ def main() -> int:
     }
     if args.ensemble_sims > 0:
         log(f"ensemble leg: {args.ensemble_sims} simulations x {args.ensemble_steps} steps")
-        result["ensemble"] = ensemble_leg(gi, args.ensemble_sims, args.ensemble_steps)
+        ens = ensemble_leg(gi, args.ensemble_sims, args.ensemble_steps)
+        # What any batched engine could reach: the ensemble's pairs at the
+        # single-impact all-pairs kernel's own rate, with no other cost.
+        bound = args.ensemble_sims / (ens["pairs_per_step"] / (pairs / grav_s))
+        ens["bound_at_single_kernel_pair_rate_sim_steps_per_s"] = bound
+        ens["batched_fraction_of_bound"] = ens["batched_sim_steps_per_s"] / bound
+        ens["bound_speedup_vs_sequential"] = bound / ens["sequential_plain_kernel_sim_steps_per_s"]
+        result["ensemble"] = ens
         log(f"ensemble: {result['ensemble']}")
     if args.serve_requests > 0:
         log(f"serving leg: {args.serve_requests} requests, Poisson gaps of mean {args.serve_gap} s")
