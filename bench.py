@@ -59,6 +59,19 @@ def traffic_from_profiles(kernel_prefix: str):
     return None
 
 
+def cpu_model() -> str:
+    """The host CPU's model name (/proc/cpuinfo), or the platform's string."""
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.lower().startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+
+    return platform.processor() or platform.machine()
+
+
 def cpu_baseline(pos, vel, fc, n, target_s: float):
     from oracle import oracle as O
 
@@ -76,6 +89,8 @@ def cpu_baseline(pos, vel, fc, n, target_s: float):
         "value": rows * n / dt,
         "unit": "pair-int/s",
         "cores": threads,
+        "cpu_model": cpu_model(),
+        "host_cpus": os.cpu_count(),
         "kind": "port",
         "ms_per_step_extrapolated": float(n) * n / (rows * n / dt) * 1e3,
         "sample": f"rows [0,{rows}) x all {n} j of one force evaluation of the N={n} impact "

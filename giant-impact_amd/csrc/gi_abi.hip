@@ -171,6 +171,7 @@ struct DevCtx {
     Workspace ws;
     CCWork cc;
     DevBuf<double> rc_blocks, rc_sums;
+    DevBuf<float> rc_shift;  // the reference-float recenter's six shifts
     hipStream_t capture_stream = nullptr;
     hipEvent_t ev_in = nullptr, ev_out = nullptr;
     // side stream of a force evaluation (default priority): the contact grid
@@ -892,6 +893,20 @@ static int launch_recenter(DevCtx* ctx, float4* pos, float4* vel, int n, int nfe
     return 0;
 }
 
+// The reference's float recenter (k_recenter_ref_sum: its host sums in its
+// order, one workgroup) and the shift of every particle.
+static int launch_recenter_ref(DevCtx* ctx, float4* pos, float4* vel, int n, int nfe, double mfe, double msi,
+                               const float* tc, const float* tv, hipStream_t st) {
+    int rc;
+    if ((rc = ctx->rc_shift.ensure(8))) return rc;
+    hipLaunchKernelGGL(k_recenter_ref_sum, dim3(1), dim3(256), 0, st, (const float4*)pos, (const float4*)vel, n, nfe,
+                       mfe, msi, tc[0], tc[1], tc[2], tv[0], tv[1], tv[2], ctx->rc_shift.p);
+    hipLaunchKernelGGL(k_recenter_ref_apply, dim3(nblk(n, 256)), dim3(256), 0, st, pos, vel, n,
+                       (const float*)ctx->rc_shift.p);
+    GI_HIP(hipGetLastError());
+    return 0;
+}
+
 static hipStream_t as_stream(gi_stream_t s) { return (hipStream_t)s; }
 
 static SimDesc make_sim(int start, int fe_end, int end, const PairLaw& L) {
@@ -1328,6 +1343,17 @@ int gi_recenter(gi_float4* pos, gi_float4* vel, int n, int nfe, double mass_fe, 
                            as_stream(stream));
 }
 
+int gi_recenter_float(gi_float4* pos, gi_float4* vel, int n, int nfe, double mass_fe, double mass_si,
+                      const float* initial_com3, const float* initial_vel3, gi_stream_t stream) {
+    if (n <= 0 || !pos || !vel || !initial_com3 || !initial_vel3 || nfe < 0 || nfe > n)
+        return set_error(GI_E_INVALID_ARG, "gi_recenter_float: invalid arguments");
+    DevCtx* ctx;
+    int rc = current_ctx(&ctx);
+    if (rc) return rc;
+    return launch_recenter_ref(ctx, (float4*)pos, (float4*)vel, n, nfe, mass_fe, mass_si, initial_com3, initial_vel3,
+                               as_stream(stream));
+}
+
 int gi_find_earth_moon(const gi_float4* pos, const gi_float4* vel, int n, int nfe1, int nfe2, int nsi1,
                        double mass_fe, double mass_si, float touch, int* group, gi_earth_moon* out,
                        gi_stream_t stream) {
@@ -1456,15 +1482,20 @@ int gi_conserved_quantities(const gi_float4* pos, const gi_float4* vel, int n, c
     return 0;
 }
 
-int gi_impact_steps(gi_float4* pos, gi_float4* vel, gi_float4* force, int n, const gi_force_collision_constants* fc,
-                    const gi_move_collision_constants* mc, int n_steps, int recenter_rate, const double* target_com3,
-                    const double* target_vel3, gi_stream_t stream) {
-    int rc = check_particles("gi_impact_steps", pos, vel, force, n);
-    if (rc) return rc;
-    if (!fc || !mc || n_steps < 0 || recenter_rate < 0 || fc->NFe < 0 || fc->NFe > n || mc->NFe != fc->NFe)
-        return set_error(GI_E_INVALID_ARG, "gi_impact_steps: invalid arguments");
-    if (recenter_rate > 0 && (!target_com3 || !target_vel3))
-        return set_error(GI_E_INVALID_ARG, "gi_impact_steps: recenter needs target_com3 and target_vel3");
+// The recenter of an impact step run: the reference's float one (its host
+// sums restated on the device, with its double element masses and float
+// targets), or FP64 device moments toward double targets.
+struct RecenterSpec {
+    bool ref_float = false;
+    double mass_fe = 0.0, mass_si = 0.0;  // ref_float: the element masses
+    float tcf[3] = {0, 0, 0}, tvf[3] = {0, 0, 0};
+    double tcd[3] = {0, 0, 0}, tvd[3] = {0, 0, 0};
+};
+
+static int impact_steps_impl(gi_float4* pos, gi_float4* vel, gi_float4* force, int n,
+                             const gi_force_collision_constants* fc, const gi_move_collision_constants* mc,
+                             int n_steps, int recenter_rate, const RecenterSpec& rs, gi_stream_t stream) {
+    int rc;
     if (n == 0 || n_steps == 0) return 0;
     DevCtx* ctx;
     if ((rc = current_ctx(&ctx))) return rc;
@@ -1521,27 +1552,43 @@ int gi_impact_steps(gi_float4* pos, gi_float4* vel, gi_float4* force, int n, con
     }
     hipGraphExec_t rec = nullptr;
     if (recenter_rate > 0) {
-        // The recenter uses the move's masses: m = Dt / (Dt/m).
-        const double mfe = (double)m.Dt / (double)m.DtOverMassFe;
-        const double msi = (double)m.Dt / (double)m.DtOverMassSi;
         std::vector<unsigned char> k2;
-        key_append(k2, 2);
+        key_append(k2, rs.ref_float ? 5 : 2);
         key_append(k2, P);
         key_append(k2, V);
         key_append(k2, n);
         key_append(k2, fc->NFe);
-        for (int a = 0; a < 3; ++a) {
-            key_append(k2, target_com3[a]);
-            key_append(k2, target_vel3[a]);
+        if (rs.ref_float) {
+            key_append(k2, rs.mass_fe);
+            key_append(k2, rs.mass_si);
+            for (int a = 0; a < 3; ++a) {
+                key_append(k2, rs.tcf[a]);
+                key_append(k2, rs.tvf[a]);
+            }
+            if ((rc = ctx->rc_shift.ensure(8))) return rc;  // scratch exists before capture
+            rc = get_graph(ctx, ctx->graphs, k2, ctx->ws.generation,
+                           [&](hipStream_t cs) -> int {
+                               return launch_recenter_ref(ctx, P, V, n, fc->NFe, rs.mass_fe, rs.mass_si, rs.tcf,
+                                                          rs.tvf, cs);
+                           },
+                           &rec, {step, steps});
+        } else {
+            // The FP64 recenter uses the move's masses: m = Dt / (Dt/m).
+            const double mfe = (double)m.Dt / (double)m.DtOverMassFe;
+            const double msi = (double)m.Dt / (double)m.DtOverMassSi;
+            for (int a = 0; a < 3; ++a) {
+                key_append(k2, rs.tcd[a]);
+                key_append(k2, rs.tvd[a]);
+            }
+            // make sure the recenter scratch exists before capture
+            if ((rc = ctx->rc_blocks.ensure((size_t)nblk(n, 256) * kMoments1 + 1))) return rc;
+            if ((rc = ctx->rc_sums.ensure(kMoments1 + 1))) return rc;
+            rc = get_graph(ctx, ctx->graphs, k2, ctx->ws.generation,
+                           [&](hipStream_t cs) -> int {
+                               return launch_recenter(ctx, P, V, n, fc->NFe, mfe, msi, rs.tcd, rs.tvd, cs);
+                           },
+                           &rec, {step, steps});
         }
-        // make sure the recenter scratch exists before capture
-        if ((rc = ctx->rc_blocks.ensure((size_t)nblk(n, 256) * kMoments1 + 1))) return rc;
-        if ((rc = ctx->rc_sums.ensure(kMoments1 + 1))) return rc;
-        rc = get_graph(ctx, ctx->graphs, k2, ctx->ws.generation,
-                       [&](hipStream_t cs) -> int {
-                           return launch_recenter(ctx, P, V, n, fc->NFe, mfe, msi, target_com3, target_vel3, cs);
-                       },
-                       &rec, {step, steps});
         if (rc) return rc;
     }
     // as steps 1..n_steps with a recenter after every step s with
@@ -1553,6 +1600,47 @@ int gi_impact_steps(gi_float4* pos, gi_float4* vel, gi_float4* force, int n, con
         if (rec && s % recenter_rate == 0) GI_HIP(hipGraphLaunch(rec, br.run));
     }
     return br.end();
+}
+
+int gi_impact_steps(gi_float4* pos, gi_float4* vel, gi_float4* force, int n, const gi_force_collision_constants* fc,
+                    const gi_move_collision_constants* mc, int n_steps, int recenter_rate, const double* target_com3,
+                    const double* target_vel3, gi_stream_t stream) {
+    int rc = check_particles("gi_impact_steps", pos, vel, force, n);
+    if (rc) return rc;
+    if (!fc || !mc || n_steps < 0 || recenter_rate < 0 || fc->NFe < 0 || fc->NFe > n || mc->NFe != fc->NFe)
+        return set_error(GI_E_INVALID_ARG, "gi_impact_steps: invalid arguments");
+    if (recenter_rate > 0 && (!target_com3 || !target_vel3))
+        return set_error(GI_E_INVALID_ARG, "gi_impact_steps: recenter needs target_com3 and target_vel3");
+    RecenterSpec rs;
+    if (recenter_rate > 0)
+        for (int a = 0; a < 3; ++a) {
+            rs.tcd[a] = target_com3[a];
+            rs.tvd[a] = target_vel3[a];
+        }
+    return impact_steps_impl(pos, vel, force, n, fc, mc, n_steps, recenter_rate, rs, stream);
+}
+
+int gi_impact_steps_float_recenter(gi_float4* pos, gi_float4* vel, gi_float4* force, int n,
+                                   const gi_force_collision_constants* fc, const gi_move_collision_constants* mc,
+                                   int n_steps, int recenter_rate, const float* initial_com3,
+                                   const float* initial_vel3, double mass_fe, double mass_si, gi_stream_t stream) {
+    int rc = check_particles("gi_impact_steps_float_recenter", pos, vel, force, n);
+    if (rc) return rc;
+    if (!fc || !mc || n_steps < 0 || recenter_rate < 0 || fc->NFe < 0 || fc->NFe > n || mc->NFe != fc->NFe)
+        return set_error(GI_E_INVALID_ARG, "gi_impact_steps_float_recenter: invalid arguments");
+    if (recenter_rate > 0 && (!initial_com3 || !initial_vel3))
+        return set_error(GI_E_INVALID_ARG, "gi_impact_steps_float_recenter: recenter needs initial_com3 and initial_vel3");
+    RecenterSpec rs;
+    rs.ref_float = true;
+    rs.mass_fe = mass_fe;
+    rs.mass_si = mass_si;
+    if (recenter_rate > 0)
+        for (int a = 0; a < 3; ++a) {
+            rs.tcf[a] = initial_com3[a];
+            rs.tvf[a] = initial_vel3[a];
+        }
+    return impact_steps_impl(pos, vel, force, n, fc, mc, n_steps, recenter_rate, rs,
+                             stream);
 }
 
 int gi_separate_steps(gi_float4* pos, gi_float4* vel, gi_float4* force, int n, const gi_force_separate_constants* fc,

@@ -1369,6 +1369,78 @@ __global__ void k_recenter_apply(float4* __restrict__ pos, float4* __restrict__ 
     vel[i] = v;
 }
 
+// The reference's own recenter (impactAddTime.cu:433-455), float for float.
+// getMassCollision(0) / getCenterOfMassCollision(0) /
+// getLinearVelocityCollision(0) (impactBasicFunctions.h:219-230,268-297,
+// 388-417) accumulate in float on the host, particle by particle in index
+// order: acc = float(double(acc) + double(x) * m) with m the double element
+// mass (iron for i < nfe), and the mass the same way with x = 1.  Each of
+// those seven sums is one sequential chain, so one workgroup stages the
+// particles through LDS a tile at a time (coalesced loads by all its lanes)
+// and seven lanes of its first wavefront run the chains side by side.  The
+// result is written as the six float shifts target - mean that every
+// particle then adds (k_recenter_ref_apply): bitwise the reference's host
+// loop.
+constexpr int kRecenterTile = 1024;
+
+__global__ __launch_bounds__(256) void k_recenter_ref_sum(const float4* __restrict__ pos,
+                                                          const float4* __restrict__ vel, int n, int nfe,
+                                                          double mass_fe, double mass_si, float tcx, float tcy,
+                                                          float tcz, float tvx, float tvy, float tvz,
+                                                          float* __restrict__ shift) {
+    __shared__ float tile[6][kRecenterTile];
+    const int lane = threadIdx.x;
+    float acc = 0.0f;  // lanes 0..5: sum m * (x, y, z, vx, vy, vz); lane 6: sum m
+    for (int t0 = 0; t0 < n; t0 += kRecenterTile) {
+        const int cnt = min(kRecenterTile, n - t0);
+        __syncthreads();  // the previous tile's chains are done
+        for (int k = lane; k < cnt; k += blockDim.x) {
+            const float4 p = pos[t0 + k];
+            const float4 v = vel[t0 + k];
+            tile[0][k] = p.x;
+            tile[1][k] = p.y;
+            tile[2][k] = p.z;
+            tile[3][k] = v.x;
+            tile[4][k] = v.y;
+            tile[5][k] = v.z;
+        }
+        __syncthreads();
+        if (lane < 7) {
+            for (int k = 0; k < cnt; ++k) {
+                const double m = t0 + k < nfe ? mass_fe : mass_si;
+                const double x = lane < 6 ? (double)tile[lane][k] : 1.0;
+                acc = (float)((double)acc + x * m);
+            }
+        }
+    }
+    __shared__ float sums[7];
+    if (lane < 7) sums[lane] = acc;
+    __syncthreads();
+    if (lane < 6) {
+        // centerOfMass.x /= totalMass in float: the double quotient rounded
+        // once to float is the correctly rounded float quotient (53 >= 2*24+2).
+        const float mean = (float)((double)sums[lane] / (double)sums[6]);
+        const float target = lane == 0 ? tcx : lane == 1 ? tcy : lane == 2 ? tcz : lane == 3 ? tvx : lane == 4 ? tvy : tvz;
+        shift[lane] = target - mean;
+    }
+}
+
+__global__ void k_recenter_ref_apply(float4* __restrict__ pos, float4* __restrict__ vel, int n,
+                                     const float* __restrict__ shift) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 p = pos[i];
+    float4 v = vel[i];
+    p.x += shift[0];
+    p.y += shift[1];
+    p.z += shift[2];
+    v.x += shift[3];
+    v.y += shift[4];
+    v.z += shift[5];
+    pos[i] = p;
+    vel[i] = v;
+}
+
 // Explicit instantiations used by the host code.
 #define GI_GRAVITY_INST(R, POT, V) \
     template __global__ void k_gravity<R, 128, POT, V>(const float4*, const GravTask*, int, const SimDesc*, float4*);

@@ -257,6 +257,12 @@ def load() -> C.CDLL:
     _optional = {
         "gi_center_of_mass_velocity": (i, [vp, vp, i, i, C.c_double, C.c_double, P(C.c_double), vp]),
         "gi_recenter": (i, [vp, vp, i, i, C.c_double, C.c_double, P(C.c_double), P(C.c_double), vp]),
+        "gi_recenter_float": (i, [vp, vp, i, i, C.c_double, C.c_double, P(C.c_float), P(C.c_float), vp]),
+        "gi_impact_steps_float_recenter": (
+            i,
+            [vp, vp, vp, i, P(ForceCollisionConstants), P(MoveCollisionConstants), i, i, P(C.c_float), P(C.c_float),
+             C.c_double, C.c_double, vp],
+        ),
         "gi_conserved_quantities": (i, [vp, vp, i, P(ForceCollisionConstants), P(C.c_double), vp]),
         "gi_impact_steps": (
             i,
@@ -582,6 +588,20 @@ def impact_steps(P, V, F, n: int, fc, mc, steps: int, recenter_rate: int = 0, co
     v3 = (C.c_double * 3)(*vel3) if vel3 is not None else None
     call("gi_impact_steps", _p(P), _p(V), _p(F), int(n), C.byref(fc), C.byref(mc), int(steps), int(recenter_rate),
          c3, v3, _p(stream))
+
+
+def recenter_float(P, V, n: int, nfe: int, mass_fe: float, mass_si: float, com3, vel3, stream=None) -> None:
+    c3 = (C.c_float * 3)(*com3)
+    v3 = (C.c_float * 3)(*vel3)
+    call("gi_recenter_float", _p(P), _p(V), int(n), int(nfe), float(mass_fe), float(mass_si), c3, v3, _p(stream))
+
+
+def impact_steps_float_recenter(P, V, F, n: int, fc, mc, steps: int, recenter_rate: int, com3, vel3,
+                                mass_fe: float, mass_si: float = 1.0, stream=None) -> None:
+    c3 = (C.c_float * 3)(*com3)
+    v3 = (C.c_float * 3)(*vel3)
+    call("gi_impact_steps_float_recenter", _p(P), _p(V), _p(F), int(n), C.byref(fc), C.byref(mc), int(steps),
+         int(recenter_rate), c3, v3, float(mass_fe), float(mass_si), _p(stream))
 
 
 def conserved_quantities(P, V, n: int, fc, stream=None) -> np.ndarray:

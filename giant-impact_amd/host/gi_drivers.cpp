@@ -2,10 +2,12 @@
 // display: body generation (generateTargetImpactor.exe), impact set-up
 // (impactInitialization.exe) and the impact run (impactAddTime.exe), on the
 // device-resident steppers of this library and the reference's file formats.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <limits>
 #include <vector>
 
 #include "../../include/giant_impact_host.h"
@@ -71,6 +73,141 @@ int write_centering(const std::string &path, const float c[6]) {
     static const char *names[6] = {"InitialCenterOfMassX",   "InitialCenterOfMassY",   "InitialCenterOfMassZ",
                                    "InitialLinearVelocityX", "InitialLinearVelocityY", "InitialLinearVelocityZ"};
     for (int k = 0; k < 6; ++k) std::fprintf(f, "\n %s = %f\n", names[k], c[k]);
+    return std::fclose(f) == 0 ? 0 : GI_E_IO;
+}
+
+// initialStatsFile: the text, order and unit conversions of
+// recordInitialImpactStat (impactInitialization.cu:477-513).  mass, com, vel
+// and L are the whole system's, in simulation units.
+int write_initial_stats(const std::string &path, const gi_units &u, double mass, const double com[3],
+                        const double vel[3], const double L[3]) {
+    FILE *f = std::fopen(path.c_str(), "wb");
+    if (!f) return GI_E_IO;
+    const double mass_c = u.unit_mass, length_c = u.unit_length, velocity_c = u.unit_length / u.unit_time;
+    const double momentum_c = u.unit_mass * u.unit_length * u.unit_length / u.unit_time;
+    std::fprintf(f, "\n\n\n*************************************************************************\n\n");
+    std::fprintf(f, "\nThe following are the initial stats of the impact\n");
+    std::fprintf(f, "\nDistance is measured in Kilometers");
+    std::fprintf(f, "\nMass is measured in Kilograms");
+    std::fprintf(f, "\nTime is measured in seconds");
+    std::fprintf(f, "\nVelocity is measured in Kilometers/second");
+    std::fprintf(f, "\nAngular momentun is measured in Kilograms*Kilometers*Kilometers/seconds\n");
+    std::fprintf(f, "\nCenter of mass of the entire system \t\t        = (%f, %f, %f)\n", com[0] * length_c,
+                 com[1] * length_c, com[2] * length_c);
+    std::fprintf(f, "\nTotal mass of the entire system \t= %e\n", mass * mass_c);
+    double x = vel[0] * velocity_c, y = vel[1] * velocity_c, z = vel[2] * velocity_c;
+    std::fprintf(f, "\nLinear velocity of the entire system system \t\t= (%f, %f, %f)", x, y, z);
+    std::fprintf(f, "\nMagnitude of the linear velocity of the entire system \t= %f\n", std::sqrt(x * x + y * y + z * z));
+    x = L[0] * momentum_c;
+    y = L[1] * momentum_c;
+    z = L[2] * momentum_c;
+    std::fprintf(f, "\nAngular momentum of the entire system system \t\t= (%e, %e, %e)", x, y, z);
+    std::fprintf(f, "\nMagnitude of the angular momentum of the entire system \t= %e\n", std::sqrt(x * x + y * y + z * z));
+    std::fprintf(f, "\n\n*************************************************************************\n\n\n");
+    return std::fclose(f) == 0 ? 0 : GI_E_IO;
+}
+
+// getAngularMomentum of a body (iron [a, b), silicate [b, c)): about its
+// centre of mass and relative to its mean velocity, both divided by the
+// nominal body mass (generateTargetImpactorBasicFunctions.h:173-237).
+void body_angmom(const gi_float4 *pos, const gi_float4 *vel, int a, int b, int c, double mass_fe, double mass_si,
+                 double total, double out[3]) {
+    double cm[3], um[3];
+    body_mean(pos, a, b, c, mass_fe, mass_si, total, cm);
+    body_mean(vel, a, b, c, mass_fe, mass_si, total, um);
+    out[0] = out[1] = out[2] = 0.0;
+    for (int i = a; i < c; ++i) {
+        const double m = i < b ? mass_fe : mass_si;
+        const double rx = pos[i].x - cm[0], ry = pos[i].y - cm[1], rz = pos[i].z - cm[2];
+        const double vx = vel[i].x - um[0], vy = vel[i].y - um[1], vz = vel[i].z - um[2];
+        out[0] += (ry * vz - rz * vy) * m;
+        out[1] += -(rx * vz - rz * vx) * m;
+        out[2] += (rx * vy - ry * vx) * m;
+    }
+}
+
+// targetImpactorStats: the text, order and conversions of
+// recordTargetImpactorStats (generateTargetImpactorBasicFunctions.h:590-686)
+// for the final bodies (layout [Fe1, Si1, Fe2, Si2]).  The body radius is the
+// largest distance of an element from the body's centre of mass (the
+// reference's getRadiusOfBody starts that maximum from an uninitialised
+// value, :87; here from 0).
+int write_target_impactor_stats(const std::string &path, const gi_generator_setup &s, const gi_units &u,
+                                const gi_float4 *pos, const gi_float4 *vel) {
+    FILE *f = std::fopen(path.c_str(), "wb");
+    if (!f) return GI_E_IO;
+    const double length_c = u.unit_length, mass_c = u.unit_mass, velocity_c = u.unit_length / u.unit_time;
+    const double momentum_c = (u.unit_mass * u.unit_length * u.unit_length) / (u.unit_time);
+    std::fprintf(f, "The conversion parameters to take you to and from our units to the real world units follow\n");
+    std::fprintf(f, "\n Simulation length unit is %f kilometers", u.unit_length);
+    std::fprintf(f, "\n Simulation mass unit is %e kilograms", u.unit_mass);
+    std::fprintf(f, "\n Simulation time unit is %f seconds", u.unit_time);
+    std::fprintf(f, "\n\n The initail statistics for this run in Simulation units follow\n");
+    std::fprintf(f, "\n Diameter of an element: \t\t\tDiameter = %f", u.diameter);
+    std::fprintf(f, "\n Gravitational constant: \t\t\tGravity = %f", u.gravity);
+    std::fprintf(f, "\n The mass of a silicate element: \tMassSi = %f", u.mass_si);
+    std::fprintf(f, "\n The mass of an iron element: \t\tMassFe = %f", u.mass_fe);
+    std::fprintf(f, "\n\n The push back strength of iron: \tKFe = %f", u.k_fe);
+    std::fprintf(f, "\n The push back strength of silicate: \tKSi = %f\n", u.k_si);
+    std::fprintf(f, "\n The mass of body one: \tMassOfBody1 = %f", u.mass_of_body1);
+    std::fprintf(f, "\n The mass of body two: \tMassOfBody2 = %f\n", u.mass_of_body2);
+    const float *sp[2] = {s.initial_spin1, s.initial_spin2};
+    const float sw[2] = {u.spin1_w, u.spin2_w};
+    static const char *which[2] = {"one", "two"};
+    for (int b = 0; b < 2; ++b) {
+        // the direction's norm from float products, as the float4 fields give it
+        const float q = sp[b][0] * sp[b][0] + sp[b][1] * sp[b][1] + sp[b][2] * sp[b][2];
+        const double mag = std::sqrt((double)q);
+        std::fprintf(f, "\n The initial spin in revolutions per time unit of body %s: (%f, %f, %f, %f)%s", which[b],
+                     sp[b][0] / mag, sp[b][1] / mag, sp[b][2] / mag, (double)sw[b], b == 1 ? "\n" : "");
+    }
+    const int nfe = u.nfe1 + u.nfe2, nsi = u.nsi1 + u.nsi2;
+    std::fprintf(f, "\n Total number of elements: \t\t\t\t\t\tTotalNumberOfElements = %d", nfe + nsi);
+    std::fprintf(f, "\n Total number of iron elements: \t\t\t\t\tNFe = %d", nfe);
+    std::fprintf(f, "\n Total number of silicate elements: \t\t\t\tNSi = %d", nsi);
+    std::fprintf(f, "\n Total number of iron elements in body1: \t\t\tNFe1 = %d", u.nfe1);
+    std::fprintf(f, "\n Total number of silicate elements in body1: \t\tNSi1 = %d", u.nsi1);
+    std::fprintf(f, "\n Total number of iron elements in body2 \t\t\tNFe2: = %d", u.nfe2);
+    std::fprintf(f, "\n Total number of silicate elements in body2: \t\tNSi2 = %d\n", u.nsi2);
+    std::fprintf(f, "\n Time step in Simulation units: \t\tDt = %f", s.dt);
+    std::fprintf(f, "\n Time step in seconds:\t\t\t\t\tDt = %f", s.dt * u.unit_time);
+    std::fprintf(f, "\n Damp time in Simulation units: \t\tDampTime = %f", u.damp_time);
+    std::fprintf(f, "\n Damp rest time in Simulation units: \tDampRestTime = %f", u.damp_rest_time);
+    std::fprintf(f, "\n Spin rest time in Simulation units: \tSpinRestTime = %f", u.spin_rest_time);
+    std::fprintf(f, "\n\n\n*****************************************************************************************************");
+    std::fprintf(f, "\nThe follow are the statistics of the final created bodies in real world units");
+    std::fprintf(f, "\n*****************************************************************************************************");
+    const int b1 = u.nfe1, b2 = u.nfe1 + u.nsi1, b3 = b2 + u.nfe2, n = b3 + u.nsi2;
+    const int ranges[2][3] = {{0, b1, b2}, {b2, b3, n}};
+    const double nominal[2] = {u.mass_of_body1, u.mass_of_body2};
+    const int counts[2][2] = {{u.nfe1, u.nsi1}, {u.nfe2, u.nsi2}};
+    for (int b = 0; b < 2; ++b) {
+        const int a0 = ranges[b][0], a1 = ranges[b][1], a2 = ranges[b][2];
+        double com[3], v[3], L[3];
+        body_mean(pos, a0, a1, a2, u.mass_fe, u.mass_si, nominal[b], com);
+        double radius = 0.0;
+        for (int i = a0; i < a2; ++i) {
+            const double rx = (double)pos[i].x - com[0], ry = (double)pos[i].y - com[1], rz = (double)pos[i].z - com[2];
+            radius = std::max(radius, std::sqrt(rx * rx + ry * ry + rz * rz));
+        }
+        const double mass = u.mass_fe * (double)counts[b][0] + u.mass_si * (double)counts[b][1];
+        std::fprintf(f, "\n\n ***** Stats for Body%d *****", b + 1);
+        std::fprintf(f, "\n Mass =  %e Kilograms", mass * mass_c);
+        std::fprintf(f, "\n Radius =  %f Kilometers", radius * length_c);
+        std::fprintf(f, "\n The center of mass = (%f, %f, %f) Kilometers from (0, 0, 0)", com[0] * length_c,
+                     com[1] * length_c, com[2] * length_c);
+        body_mean(vel, a0, a1, a2, u.mass_fe, u.mass_si, nominal[b], v);
+        std::fprintf(f, "\n The average linear velocity = (%f, %f, %f) Kilometers/second", v[0] * velocity_c,
+                     v[1] * velocity_c, v[2] * velocity_c);
+        std::fprintf(f, "\n The magitude of the avergae linear velocity = %f Kilometers/second",
+                     std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) * velocity_c);
+        body_angmom(pos, vel, a0, a1, a2, u.mass_fe, u.mass_si, nominal[b], L);
+        std::fprintf(f, "\n The angular momentum = (%e, %e, %e) Kilograms*kilometers*kilometers/second",
+                     L[0] * momentum_c, L[1] * momentum_c, L[2] * momentum_c);
+        std::fprintf(f, "\n The magitude of the angular momentum = %e Kilograms*kilometers*kilometers/second",
+                     std::sqrt(L[0] * L[0] + L[1] * L[1] + L[2] * L[2]) * momentum_c);
+    }
+    std::fprintf(f, "\n\n");
     return std::fclose(f) == 0 ? 0 : GI_E_IO;
 }
 
@@ -312,6 +449,7 @@ int gi_generate_bodies(const gi_generator_setup *setup, unsigned long long seed,
     if (out_dir) {
         // recordCarryForwardParameters / recordTargetImpactorInitialPosVel
         // (generateTargetImpactorBasicFunctions.h:688-722).
+        if ((rc = write_target_impactor_stats(join(out_dir, "targetImpactorStats"), *setup, u, pos, vel))) return rc;
         if ((rc = gi_parameters_write(join(out_dir, "targetImpactorParameters").c_str(), &u, n))) return rc;
         FILE *f = std::fopen(join(out_dir, "targetImpactorInitialPosVel").c_str(), "wb");
         if (!f) return GI_E_IO;
@@ -410,8 +548,21 @@ int gi_impact_initialize(const char *in_dir, const char *setup_path, const char 
         c[4] += vel[i].y * m;
         c[5] += vel[i].z * m;
     }
+    for (int q = 0; q < 6; ++q) c[q] /= mt;
+    // Angular momentum about the centre of mass, relative to the mean
+    // velocity, in double (impactInitialization.cu:448-475).
+    double L[3] = {0.0, 0.0, 0.0};
+    for (int i = 0; i < n; ++i) {
+        const double m = i < nfe ? u.mass_fe : u.mass_si;
+        const double rx = pos[i].x - c[0], ry = pos[i].y - c[1], rz = pos[i].z - c[2];
+        const double vx = vel[i].x - c[3], vy = vel[i].y - c[4], vz = vel[i].z - c[5];
+        L[0] += (ry * vz - rz * vy) * m;
+        L[1] += -(rx * vz - rz * vx) * m;
+        L[2] += (rx * vy - ry * vx) * m;
+    }
+    if ((rc = write_initial_stats(join(out_dir, "initialStatsFile"), u, mt, c, c + 3, L))) return rc;
     float cf[6];
-    for (int q = 0; q < 6; ++q) cf[q] = (float)(c[q] / mt);
+    for (int q = 0; q < 6; ++q) cf[q] = (float)c[q];
     return write_centering(join(out_dir, "viewCenteringInfo"), cf);
 }
 
@@ -432,7 +583,9 @@ int gi_impact_run(const char *param_dir, const char *setup_path, const char *run
               r.next(recenter_rate)))
             return GI_E_PARSE;
     }
-    if (dt <= 0.0 || record_rate <= 0 || recenter_rate <= 0) return GI_E_INVALID_ARG;
+    // A rate of zero or less never fires: the reference's counters start at 1
+    // after the first step and never equal it (impactAddTime.cu:402-404,420,433).
+    if (!(dt > 0.0)) return GI_E_INVALID_ARG;
     additional_time *= 24.0 * 60.0 * 60.0;
     additional_time /= u.unit_time;
     float centering[6];
@@ -484,22 +637,21 @@ int gi_impact_run(const char *param_dir, const char *setup_path, const char *run
         std::fclose(ff);
         return rc;
     }
-    const double tc[3] = {centering[0], centering[1], centering[2]};
-    const double tv[3] = {centering[3], centering[4], centering[5]};
     const int nfe = u.nfe1 + u.nfe2;
+    const long long never = std::numeric_limits<long long>::max();
     int frames_written = 0;
     long long done = 0;
     const double start_time = run_time;
     while (done < n_steps) {
         // run to the next record / recentre event or the end
-        const long long next_record = (done / record_rate + 1) * record_rate;
-        const long long next_recenter = (done / recenter_rate + 1) * recenter_rate;
+        const long long next_record = record_rate > 0 ? (done / record_rate + 1) * record_rate : never;
+        const long long next_recenter = recenter_rate > 0 ? (done / recenter_rate + 1) * recenter_rate : never;
         const long long target = std::min(std::min(next_record, next_recenter), n_steps);
         const int k = (int)(target - done);
         if ((rc = gi_impact_steps(P, V, F, n, &fc, &mc, k, 0, nullptr, nullptr, nullptr))) break;
         for (int q = 0; q < k; ++q) run_time += dt;
         done = target;
-        if (done % record_rate == 0) {
+        if (record_rate > 0 && done % record_rate == 0) {
             if ((rc = gi_device_sync())) break;
             if ((rc = gi_copy_d2h(pos.data(), P, bytes)) || (rc = gi_copy_d2h(vel.data(), V, bytes))) break;
             const float tf = (float)run_time;
@@ -510,8 +662,11 @@ int gi_impact_run(const char *param_dir, const char *setup_path, const char *run
             }
             ++frames_written;
         }
-        if (done % recenter_rate == 0) {
-            if ((rc = gi_recenter(P, V, n, nfe, u.mass_fe, u.mass_si, tc, tv, nullptr))) break;
+        if (recenter_rate > 0 && done % recenter_rate == 0) {
+            // the reference's float recenter toward the viewCenteringInfo
+            // floats, with the %f-read element masses
+            if ((rc = gi_recenter_float(P, V, n, nfe, u.mass_fe, u.mass_si, centering, centering + 3, nullptr)))
+                break;
         }
     }
     if (!rc) rc = gi_device_sync();

@@ -205,6 +205,18 @@ int gi_center_of_mass_velocity(const gi_float4 *pos, const gi_float4 *vel, int n
 int gi_recenter(gi_float4 *pos, gi_float4 *vel, int n, int nfe, double mass_fe, double mass_si,
                 const double *target_com3, const double *target_vel3, gi_stream_t stream);
 
+/* The reference's own recenter, float for float (impactAddTime.cu:433-455
+ * with getMassCollision(0) / getCenterOfMassCollision(0) /
+ * getLinearVelocityCollision(0), impactBasicFunctions.h:219-230,268-297,
+ * 388-417): the whole system's mass, centre of mass and mean velocity summed
+ * in float particle by particle in index order (each term formed in double
+ * with the double element masses mass_fe / mass_si, iron for id < nfe), the
+ * means divided in float, then pos += initial_com3 - com and
+ * vel += initial_vel3 - mean_vel in float.  Runs on the device (no host round
+ * trip); the result is bitwise what the reference's host loop produces. */
+int gi_recenter_float(gi_float4 *pos, gi_float4 *vel, int n, int nfe, double mass_fe, double mass_si,
+                      const float *initial_com3, const float *initial_vel3, gi_stream_t stream);
+
 /* FP64 conserved quantities of one simulation (particles [0,n), iron id < nfe):
  * out[0] kinetic, out[1] potential, out[2] total energy,
  * out[3..5] linear momentum, out[6..8] angular momentum about the centre of mass.
@@ -226,6 +238,18 @@ int gi_impact_steps(gi_float4 *pos, gi_float4 *vel, gi_float4 *force, int n,
                     const gi_force_collision_constants *fc, const gi_move_collision_constants *mc,
                     int n_steps, int recenter_rate, const double *target_com3,
                     const double *target_vel3, gi_stream_t stream);
+
+/* nBodyCollisionSingleGPU's step and recenter bookkeeping exactly
+ * (impactAddTime.cu:396-455): n_steps of (force, move), and after every
+ * recenter_rate-th step (0 = never) the reference's float recenter
+ * (gi_recenter_float) toward initial_com3 / initial_vel3 -- the
+ * InitialCenterOfMass / InitialLinearVelocity floats of viewCenteringInfo --
+ * with the %f-read element masses mass_fe / mass_si.  Graph-captured like
+ * gi_impact_steps; the drop-in form of the reference's loop. */
+int gi_impact_steps_float_recenter(gi_float4 *pos, gi_float4 *vel, gi_float4 *force, int n,
+                                   const gi_force_collision_constants *fc, const gi_move_collision_constants *mc,
+                                   int n_steps, int recenter_rate, const float *initial_com3,
+                                   const float *initial_vel3, double mass_fe, double mass_si, gi_stream_t stream);
 
 /* The generation phases' step (generateTargetImpactor.cu:670,684,696,711):
  * n_steps of (gi_force_separate, gi_move_damped_separate with damp1/damp2),

@@ -692,6 +692,128 @@ void or_body_vel(const or_float4 *vel, int start_fe, int stop_fe, int stop_si, d
     or_body_com(vel, start_fe, stop_fe, stop_si, mass_fe, mass_si, total_mass, out4);
 }
 
+/* getAngularMomentum, generateTargetImpactorBasicFunctions.h:173-237: about
+ * the body's centre of mass and relative to its mean velocity (both divided
+ * by the nominal total_mass), iron then silicate.  out4 = x,y,z,|.| */
+void or_body_angmom(const or_float4 *pos, const or_float4 *vel, int start_fe, int stop_fe, int stop_si,
+                    double mass_fe, double mass_si, double total_mass, double *out4)
+{
+    double c[4], u[4];
+    or_body_com(pos, start_fe, stop_fe, stop_si, mass_fe, mass_si, total_mass, c);
+    or_body_vel(vel, start_fe, stop_fe, stop_si, mass_fe, mass_si, total_mass, u);
+    double lx = 0.0, ly = 0.0, lz = 0.0;
+    for (int i = start_fe; i < stop_si; i++) {
+        double m = (i < stop_fe) ? mass_fe : mass_si;
+        double rx = pos[i].x - c[0], ry = pos[i].y - c[1], rz = pos[i].z - c[2];
+        double vx = vel[i].x - u[0], vy = vel[i].y - u[1], vz = vel[i].z - u[2];
+        lx += (ry * vz - rz * vy) * m;
+        ly += -(rx * vz - rz * vx) * m;
+        lz += (rx * vy - ry * vx) * m;
+    }
+    out4[0] = lx;
+    out4[1] = ly;
+    out4[2] = lz;
+    out4[3] = sqrt(lx * lx + ly * ly + lz * lz);
+}
+
+/* getRadiusOfBody, generateTargetImpactorBasicFunctions.h:82-117: the largest
+ * distance of an element of [start, stop) from the body's centre of mass
+ * (nominal total_mass).  The reference starts from an uninitialised radius
+ * (:87); here from 0. */
+double or_body_radius(const or_float4 *pos, int start, int stop_fe, int stop, double mass_fe, double mass_si,
+                      double total_mass)
+{
+    double c[4];
+    or_body_com(pos, start, stop_fe, stop, mass_fe, mass_si, total_mass, c);
+    double radius = 0.0;
+    for (int i = start; i < stop; i++) {
+        double rx = (double)pos[i].x - c[0], ry = (double)pos[i].y - c[1], rz = (double)pos[i].z - c[2];
+        double d = sqrt(rx * rx + ry * ry + rz * rz);
+        if (d > radius) radius = d;
+    }
+    return radius;
+}
+
+/* impactInitialization.exe's set-up (impactInitialization.cu:235-249 units of
+ * the offsets; :300-371 offsets added per body and the reorder
+ * [Fe1, Si1, Fe2, Si2] -> [Fe1, Fe2, Si1, Si2]; :393-446 the whole system's
+ * mass, centre of mass, mean velocity and angular momentum in double).
+ * ip / iv: the setup file's offsets in km and km/s as read (float).
+ * stats10 = {mass, com xyz, vel xyz, L xyz} in simulation units. */
+void or_impact_initialize(const or_float4 *gpos, const or_float4 *gvel, int nfe1, int nsi1, int nfe2, int nsi2,
+                          const float *ip1, const float *ip2, const float *iv1, const float *iv2,
+                          double unit_length, double unit_time, double mass_fe, double mass_si, or_float4 *pos,
+                          or_float4 *vel, double *stats10)
+{
+    const int n1 = nfe1 + nsi1, n = n1 + nfe2 + nsi2, nfe = nfe1 + nfe2;
+    float p1[3], p2[3], v1[3], v2[3];
+    for (int k = 0; k < 3; k++) {
+        p1[k] = (float)(ip1[k] / unit_length); /* float /= double */
+        p2[k] = (float)(ip2[k] / unit_length);
+        v1[k] = (float)(iv1[k] * (unit_time / unit_length)); /* float *= double */
+        v2[k] = (float)(iv2[k] * (unit_time / unit_length));
+    }
+    or_float4 *tp = (or_float4 *)malloc(sizeof(or_float4) * (size_t)(n > 0 ? n : 1));
+    or_float4 *tv = (or_float4 *)malloc(sizeof(or_float4) * (size_t)(n > 0 ? n : 1));
+    for (int i = 0; i < n; i++) {
+        const float *dp = i < n1 ? p1 : p2, *dv = i < n1 ? v1 : v2;
+        tp[i] = gpos[i];
+        tv[i] = gvel[i];
+        tp[i].x += dp[0];
+        tp[i].y += dp[1];
+        tp[i].z += dp[2];
+        tv[i].x += dv[0];
+        tv[i].y += dv[1];
+        tv[i].z += dv[2];
+    }
+    int k = 0;
+    for (int i = 0; i < nfe1; i++, k++) { pos[k] = tp[i]; vel[k] = tv[i]; }
+    for (int i = n1; i < n1 + nfe2; i++, k++) { pos[k] = tp[i]; vel[k] = tv[i]; }
+    for (int i = nfe1; i < n1; i++, k++) { pos[k] = tp[i]; vel[k] = tv[i]; }
+    for (int i = n1 + nfe2; i < n; i++, k++) { pos[k] = tp[i]; vel[k] = tv[i]; }
+    free(tp);
+    free(tv);
+    double M = 0.0, cx = 0.0, cy = 0.0, cz = 0.0, ux = 0.0, uy = 0.0, uz = 0.0;
+    for (int i = 0; i < n; i++) M += (i < nfe) ? mass_fe : mass_si;
+    for (int i = 0; i < n; i++) {
+        double m = (i < nfe) ? mass_fe : mass_si;
+        cx += pos[i].x * m;
+        cy += pos[i].y * m;
+        cz += pos[i].z * m;
+    }
+    cx /= M;
+    cy /= M;
+    cz /= M;
+    for (int i = 0; i < n; i++) {
+        double m = (i < nfe) ? mass_fe : mass_si;
+        ux += vel[i].x * m;
+        uy += vel[i].y * m;
+        uz += vel[i].z * m;
+    }
+    ux /= M;
+    uy /= M;
+    uz /= M;
+    double lx = 0.0, ly = 0.0, lz = 0.0;
+    for (int i = 0; i < n; i++) {
+        double m = (i < nfe) ? mass_fe : mass_si;
+        double rx = pos[i].x - cx, ry = pos[i].y - cy, rz = pos[i].z - cz;
+        double vx = vel[i].x - ux, vy = vel[i].y - uy, vz = vel[i].z - uz;
+        lx += (ry * vz - rz * vy) * m;
+        ly += -(rx * vz - rz * vx) * m;
+        lz += (rx * vy - ry * vx) * m;
+    }
+    stats10[0] = M;
+    stats10[1] = cx;
+    stats10[2] = cy;
+    stats10[3] = cz;
+    stats10[4] = ux;
+    stats10[5] = uy;
+    stats10[6] = uz;
+    stats10[7] = lx;
+    stats10[8] = ly;
+    stats10[9] = lz;
+}
+
 /* spinBody, generateTargetImpactorBasicFunctions.h:407-470. */
 int or_spin_body(const or_float4 *pos, or_float4 *vel, int start, int stop, int stop_fe,
                  double mass_fe, double mass_si, double total_mass, const float *spin4)

@@ -116,6 +116,19 @@ void or_body_com(const or_float4 *pos, int start_fe, int stop_fe, int stop_si, d
                  double mass_si, double total_mass, double *out4);
 void or_body_vel(const or_float4 *vel, int start_fe, int stop_fe, int stop_si, double mass_fe,
                  double mass_si, double total_mass, double *out4);
+/* getAngularMomentum (generateTargetImpactorBasicFunctions.h:173-237). out4 = x,y,z,|.| */
+void or_body_angmom(const or_float4 *pos, const or_float4 *vel, int start_fe, int stop_fe, int stop_si,
+                    double mass_fe, double mass_si, double total_mass, double *out4);
+/* getRadiusOfBody (generateTargetImpactorBasicFunctions.h:82-117), starting from 0. */
+double or_body_radius(const or_float4 *pos, int start, int stop_fe, int stop, double mass_fe, double mass_si,
+                      double total_mass);
+/* impactInitialization.cu:235-249,300-371,393-446: offsets (km, km/s as read)
+ * added per body, reorder to [Fe1, Fe2, Si1, Si2], and the system's FP64
+ * stats10 = {mass, com xyz, vel xyz, L xyz}. */
+void or_impact_initialize(const or_float4 *gpos, const or_float4 *gvel, int nfe1, int nsi1, int nfe2, int nsi2,
+                          const float *ip1, const float *ip2, const float *iv1, const float *iv2,
+                          double unit_length, double unit_time, double mass_fe, double mass_si, or_float4 *pos,
+                          or_float4 *vel, double *stats10);
 /* spinBody (generateTargetImpactorBasicFunctions.h:407-470). spin4 = direction xyz,
  * revolutions per time unit w. Returns 0, or -1 for a zero direction. */
 int or_spin_body(const or_float4 *pos, or_float4 *vel, int start, int stop, int stop_fe,

@@ -98,6 +98,9 @@ def load() -> C.CDLL:
         "or_body_com": (None, [vp, i, i, i, d, d, d, vp]),
         "or_body_vel": (None, [vp, i, i, i, d, d, d, vp]),
         "or_spin_body": (i, [vp, vp, i, i, i, d, d, d, vp]),
+        "or_body_angmom": (None, [vp, vp, i, i, i, d, d, d, vp]),
+        "or_body_radius": (d, [vp, i, i, i, d, d, d]),
+        "or_impact_initialize": (None, [vp, vp, i, i, i, i, vp, vp, vp, vp, d, d, d, d, vp, vp, vp]),
         "or_impact_steps_f32": (None, [vp, vp, vp, i, P(FC), P(MC), i, i, vp, vp, d, d, i]),
         "or_trunc_to_int": (i, [C.c_float]),
     }
@@ -271,6 +274,38 @@ def body_com(pos, start_fe, stop_fe, stop_si, mass_fe, mass_si, total_mass) -> n
     out = np.zeros(4, dtype=np.float64)
     load().or_body_com(_ptr(pos), start_fe, stop_fe, stop_si, mass_fe, mass_si, total_mass, _ptr(out))
     return out
+
+
+def body_vel(vel, start_fe, stop_fe, stop_si, mass_fe, mass_si, total_mass) -> np.ndarray:
+    vel = _f4(vel)
+    out = np.zeros(4, dtype=np.float64)
+    load().or_body_vel(_ptr(vel), start_fe, stop_fe, stop_si, mass_fe, mass_si, total_mass, _ptr(out))
+    return out
+
+
+def body_angmom(pos, vel, start_fe, stop_fe, stop_si, mass_fe, mass_si, total_mass) -> np.ndarray:
+    pos, vel = _f4(pos), _f4(vel)
+    out = np.zeros(4, dtype=np.float64)
+    load().or_body_angmom(_ptr(pos), _ptr(vel), start_fe, stop_fe, stop_si, mass_fe, mass_si, total_mass, _ptr(out))
+    return out
+
+
+def body_radius(pos, start, stop_fe, stop, mass_fe, mass_si, total_mass) -> float:
+    return float(load().or_body_radius(_ptr(_f4(pos)), start, stop_fe, stop, mass_fe, mass_si, total_mass))
+
+
+def impact_initialize(gpos, gvel, nfe1, nsi1, nfe2, nsi2, ip1, ip2, iv1, iv2, unit_length, unit_time,
+                      mass_fe, mass_si=1.0):
+    """impactInitialization's set-up restated: (pos, vel) in the impact
+    layout and stats10 = [mass, com xyz, vel xyz, L xyz] (FP64, sim units)."""
+    gpos, gvel = _f4(gpos), _f4(gvel)
+    pos, vel = np.zeros_like(gpos), np.zeros_like(gvel)
+    st = np.zeros(10, dtype=np.float64)
+    f3 = [np.ascontiguousarray(a, dtype=np.float32) for a in (ip1, ip2, iv1, iv2)]
+    load().or_impact_initialize(_ptr(gpos), _ptr(gvel), int(nfe1), int(nsi1), int(nfe2), int(nsi2),
+                                *[_ptr(a) for a in f3], float(unit_length), float(unit_time), float(mass_fe),
+                                float(mass_si), _ptr(pos), _ptr(vel), _ptr(st))
+    return pos, vel, st
 
 
 def impact_steps_f32(pos, vel, fc, mc, n_steps, recenter_rate=0, com3=(0, 0, 0), vel3=(0, 0, 0),

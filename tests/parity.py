@@ -6,10 +6,10 @@ float restatement of the reference kernel (Fref).  With cond_i = sum_j |f_ij|:
   conditioned error   max_i |F_i - F64_i| / cond_i                 <= 1e-5
   relative error      per particle, |F_i - F64_i| <= |Fref_i - F64_i| + 1e-5 |F64_i|
                           (without Fref: max_i |F_i - F64_i| / |F64_i| <= 1e-5)
-  against Fref        max_i |F_i - Fref_i| / |F64_i|
-                          <= 1e-5 + (Fref's own relative error)
-                      and the 99.9th percentile likewise (the triangle
-                      inequality with both errors over the same |F64_i|).
+  against Fref        per particle, |F_i - Fref_i| / |F64_i|
+                          <= 1e-5 + 2 (Fref_i's own relative error)
+                      (the triangle inequality with the bound above), and
+                      the 99.9th percentile <= 1e-5 + Fref's own 99.9th.
 
 The relative figures are bounded by the reference's own error because in
 contact-dominated states a particle's net force is a small difference of
@@ -25,7 +25,8 @@ conditioned error has no such escape.
 
 Particles with a pair whose branch decision lies within 1e-6 of a threshold
 (r = 1, a shell break, dp.dv = 1) are excluded: there a last-bit rounding
-difference legitimately selects another branch of the reference's law.
+difference legitimately selects another branch of the reference's law.  At
+most 0.1% of the rows may be excluded that way.
 """
 from __future__ import annotations
 
@@ -58,15 +59,23 @@ def force_metrics(F, F64, cond, near, Fref=None) -> dict:
         out["ref_cond_max"] = float(np.max(np.linalg.norm(Fr - F64, axis=1)[ok] / c[ok])) if ok.any() else 0.0
         # per particle: how far this evaluation's relative error exceeds Fref's
         out["rel_excess_max"] = float(np.max(e[ok] / m[ok] - er[ok])) if ok.any() else 0.0
+        # per particle: distance from Fref beyond 1e-5 + twice Fref's own
+        # error on that particle (the triangle inequality with the bound
+        # above), so no particle borrows another's reference error
+        out["vs_ref_excess_max"] = float(np.max(d[ok] - 2.0 * er[ok])) if ok.any() else 0.0
     return out
 
 
+NEAR_MAX_FRACTION = 1e-3  # rows a last-bit branch difference may excuse
+
+
 def assert_force_parity(met: dict, check_ref: bool = True) -> None:
+    assert met["near_threshold"] <= NEAR_MAX_FRACTION * met["n"], met
     assert met["cond_max"] <= COND_TOL, met
     if "rel_excess_max" in met:
         assert met["rel_excess_max"] <= REL_TOL, met
     else:
         assert met["rel_max"] <= REL_TOL, met
     if check_ref and "vs_ref_max" in met:
-        assert met["vs_ref_max"] <= REL_TOL + met["ref_rel_max"], met
+        assert met["vs_ref_excess_max"] <= REL_TOL, met
         assert met["vs_ref_p999"] <= REL_TOL + met["ref_rel_p999"], met

@@ -466,3 +466,200 @@ def test_final_stats_scope_one_is_the_union(oracle):
     assert np.allclose(M[1] * U[1], M[2] * U[2] + M[3] * U[3], rtol=1e-12, atol=1e-12)
     orb = sum(M[k] * np.cross(C[k] - C[1], U[k] - U[1]) for k in (2, 3))
     assert np.allclose(L[1], L[2] + L[3] + orb, rtol=1e-10, atol=1e-9 * np.abs(L[1]).max())
+
+
+# ---- generator constants (generateTargetImpactor.cu:406-451) ------------------
+
+
+def test_separate_constants_known_answers_at_2048(oracle):
+    """loadKernalConstantStructures of the generator at N = 2,048 with the
+    template defaults: masses from the density ratio alone (MassSi = 1,
+    G = 1), stiffnesses from the K_sim known answer 551.16 (units table at
+    N = 2,048), shell breaks 1 - SD, and the borders of two equal bodies
+    (0.55 Earth masses each, 26% iron): 264 irons and 1,784 silicates split
+    in halves, so [Fe1 132 | Si1 892 | Fe2 132 | Si2 892]."""
+    u = oracle.units(2048)
+    fs, ms = oracle.separate_constants(u)
+    mfe = 7.850 / 3.320  # unrounded: the generator never reads the %f file
+    assert fs.GMassFeFe == np.float32(mfe * mfe)
+    assert fs.GMassFeSi == np.float32(mfe)
+    k = 551.16
+    for name, want in (("KFeFe", 2 * k), ("KSiSi", 2 * k), ("KFeSi", 2 * k), ("KRFeFe", 2 * k * 0.01),
+                       ("KRSiSi", 2 * k * 0.01), ("KRFeSi", 2 * k * 0.01), ("KRMix", k + k * 0.01)):
+        assert getattr(fs, name) == pytest.approx(want, rel=2e-5), name
+    assert fs.ShellBreakFe == np.float32(0.98) and fs.ShellBreakSi == np.float32(0.998)
+    assert fs.ShellBreakFeSi1 == np.float32(0.998) and fs.ShellBreakFeSi2 == np.float32(0.98)
+    assert (fs.boarder1, fs.boarder2, fs.boarder3) == (132, 1024, 1156)
+    assert (ms.boarder1, ms.boarder2, ms.boarder3) == (132, 1024, 1156)
+    assert ms.Dt == np.float32(0.002)
+    assert ms.DtOverMassFe == np.float32(0.002 / mfe)
+    assert ms.DtOverMassSi == np.float32(0.002)
+
+
+def test_separate_constants_unrounded_against_rounded_impact_constants(oracle):
+    """The generator's structures use the unrounded values, the impact's the
+    %f-read ones: the same constants to within that rounding (6 decimals),
+    and for MassFe the two differ exactly as 7.85/3.32 and 2.364458 do."""
+    u = oracle.units(65536)
+    fs, _ = oracle.separate_constants(u)
+    fc, _ = oracle.impact_constants(u)
+    for name in ("KFeFe", "KSiSi", "KFeSi", "KRFeFe", "KRSiSi", "KRFeSi", "KRMix", "GMassFeFe"):
+        assert getattr(fs, name) == pytest.approx(getattr(fc, name), abs=5e-6 * 2 * u.k_fe_sim), name
+    assert fs.GMassFeSi == np.float32(7.850 / 3.320) and fc.GMassFeSi == np.float32(2.364458)
+    for name in ("ShellBreakFe", "ShellBreakSi", "ShellBreakFeSi1", "ShellBreakFeSi2"):
+        assert getattr(fs, name) == getattr(fc, name), name
+
+
+@pytest.mark.parametrize("sd", [(0.02, 0.002), (0.002, 0.02)])
+def test_separate_constants_follow_the_broken_shell(oracle, sd):
+    """Unequal iron and silicate stiffness (K_Si = 3 K_Fe, KR_Fe = 0.1,
+    KR_Si = 0.3) expose a swapped material: the pure pairs are 2K and 2K KR
+    of their own material, the mixed pair the sum of one of each, and in the
+    band between the two shell breaks only the material whose shell has
+    broken (the larger SD) takes its reduced stiffness."""
+    u = oracle.units(2048, k=(1.0e14, 3.0e14), kr=(0.1, 0.3), sd=sd)
+    fs, _ = oracle.separate_constants(u)
+    kfe, ksi = u.k_fe_sim, u.k_si_sim
+    assert ksi == pytest.approx(3 * kfe, rel=1e-12)
+    f32 = np.float32
+    assert fs.KFeFe == f32(2 * kfe) and fs.KSiSi == f32(2 * ksi) and fs.KFeSi == f32(kfe + ksi)
+    assert fs.KSiSi == pytest.approx(3 * fs.KFeFe, rel=1e-6)
+    assert fs.KRFeFe == pytest.approx(2 * kfe * 0.1, rel=1e-6)
+    assert fs.KRSiSi == pytest.approx(2 * ksi * 0.3, rel=1e-6)
+    assert fs.KRFeSi == pytest.approx(kfe * 0.1 + ksi * 0.3, rel=1e-6)
+    sd_fe, sd_si = sd
+    if sd_fe >= sd_si:  # iron's shell breaks first (outer band r < 1 - SD_Si only for silicate)
+        assert fs.KRMix == pytest.approx(kfe + ksi * 0.3, rel=1e-6)
+    else:
+        assert fs.KRMix == pytest.approx(kfe * 0.1 + ksi, rel=1e-6)
+    assert fs.ShellBreakFe == f32(1 - sd_fe) and fs.ShellBreakSi == f32(1 - sd_si)
+    assert fs.ShellBreakFeSi1 == f32(1 - min(sd)) and fs.ShellBreakFeSi2 == f32(1 - max(sd))
+
+
+def test_separate_borders_reproduce_the_mass_split(oracle):
+    """Unequal bodies (0.9 and 0.1 Earth masses, 30% and 20% iron): the
+    counts reproduce each material's mass share to within one element, each
+    body's iron and silicate share to within one element, and the borders
+    are the cumulative counts in the order [Fe1, Si1, Fe2, Si2]."""
+    n = 20000
+    u = oracle.units(n, fraction_earth_mass=(0.9, 0.1), fe=(0.3, 0.2))
+    fs, ms = oracle.separate_constants(u)
+    mfe_tot, msi_tot = 0.9 * 0.3 + 0.1 * 0.2, 0.9 * 0.7 + 0.1 * 0.8
+    ratio = 7.850 / 3.320  # element mass ratio
+    nfe, nsi = u.nfe, u.nsi
+    assert nfe + nsi == n
+    assert abs(nfe * ratio / (nfe * ratio + nsi) - mfe_tot / (mfe_tot + msi_tot)) < 2.0 / n
+    assert abs(u.nfe1 - nfe * (0.9 * 0.3) / mfe_tot) < 1.0
+    assert abs(u.nsi1 - nsi * (0.9 * 0.7) / msi_tot) < 1.0
+    assert (fs.boarder1, fs.boarder2, fs.boarder3) == (u.nfe1, u.nfe1 + u.nsi1, u.nfe1 + u.nsi1 + u.nfe2)
+    assert n - fs.boarder3 == u.nsi2
+    assert (ms.boarder1, ms.boarder2, ms.boarder3) == (fs.boarder1, fs.boarder2, fs.boarder3)
+
+
+# ---- body centre of mass / mean velocity (generateTargetImpactorBasicFunctions.h:29-80,119-171)
+
+
+def test_body_com_and_velocity_closed_form(oracle):
+    """Body 2 of a layout [Fe1 Fe1 Si1 | Fe2 Fe2 Fe2 Si2 Si2]: iron mass 2.5,
+    silicate 1.0, iron of body 2 at indices 3..5 and silicate at 6..7.  The
+    sums are divided by the NOMINAL body mass passed in (10.0, not the
+    elements' 9.5), as the reference divides by MassOfBody2."""
+    x = np.array([
+        [9, 9, 9], [9, 9, 9], [9, 9, 9],          # body 1: must not contribute
+        [1, 0, 0], [0, 2, 0], [0, 0, 4],          # Fe2
+        [2, 2, 2], [-4, 0, 1],                    # Si2
+    ], np.float32)
+    v = x[:, [2, 0, 1]] * np.float32(0.5)
+    pos, vel = f4(8), f4(8)
+    pos[:, :3], vel[:, :3] = x, v
+    # sum m x = 2.5 * (1, 2, 4) + (2, 2, 2) + (-4, 0, 1) = (0.5, 7, 13)
+    c = oracle.body_com(pos, 3, 6, 8, 2.5, 1.0, 10.0)
+    assert np.array_equal(c[:3], [0.05, 0.7, 1.3])
+    assert c[3] == pytest.approx(math.sqrt(0.05**2 + 0.7**2 + 1.3**2), rel=1e-15)
+    # velocities are (z, x, y) / 2: sum m v = (13, 0.5, 7) / 2
+    w = oracle.body_vel(vel, 3, 6, 8, 2.5, 1.0, 10.0)
+    assert np.allclose(w[:3], [0.65, 0.025, 0.35], rtol=1e-15, atol=0)
+    # an empty (massless) body: all zero, no division
+    assert np.array_equal(oracle.body_com(pos, 3, 6, 8, 2.5, 1.0, 0.0), [0, 0, 0, 0])
+    # body 1 alone (iron 0..1, silicate 2): (2.5 * 18 + 9) / 6 = 9 per axis
+    assert np.array_equal(oracle.body_com(pos, 0, 2, 3, 2.5, 1.0, 6.0)[:3], [9, 9, 9])
+
+
+def test_body_angular_momentum_and_radius_closed_forms(oracle):
+    """getAngularMomentum of a rigidly rotating, drifting body is I w about
+    its centre of mass (independent of the drift), and getRadiusOfBody is the
+    largest distance from that centre."""
+    rng = np.random.default_rng(21)
+    x = _blob(rng, 216, (4.0, -1.0, 2.0))
+    n, nfe, mfe = len(x), 60, 2.364458
+    m = np.where(np.arange(n) < nfe, mfe, 1.0)
+    c = (m[:, None] * x).sum(0) / m.sum()
+    w = np.array([0.01, 0.04, -0.02])
+    v = np.cross(w, x - c) + np.array([1.0, -2.0, 0.5])
+    pos, vel = f4(n), f4(n)
+    pos[:, :3], vel[:, :3] = x, v
+    r = pos[:, :3].astype(np.float64) - c
+    I = (m[:, None, None] * ((r * r).sum(1)[:, None, None] * np.eye(3) - r[:, :, None] * r[:, None, :])).sum(0)
+    L = oracle.body_angmom(pos, vel, 0, nfe, n, mfe, 1.0, m.sum())
+    assert np.allclose(L[:3], I @ w, rtol=2e-5, atol=2e-5 * np.abs(I @ w).max())
+    assert L[3] == pytest.approx(np.linalg.norm(L[:3]), rel=1e-14)
+    d = np.sqrt(((pos[:, :3].astype(np.float64) - oracle.body_com(pos, 0, nfe, n, mfe, 1.0, m.sum())[:3]) ** 2).sum(1))
+    assert oracle.body_radius(pos, 0, nfe, n, mfe, 1.0, m.sum()) == pytest.approx(d.max(), rel=1e-15)
+    # a lone far element sets the radius: the centre of three unit masses at
+    # (+-1, 0, 0) and (0, 0, 30) is (0, 0, 10), the far one 20 from it
+    p3 = f4(3)
+    p3[:, :3] = [[-1, 0, 0], [1, 0, 0], [0, 0, 30]]
+    assert oracle.body_radius(p3, 0, 0, 3, 2.0, 1.0, 3.0) == 20.0
+
+
+def _generator_state(rng, counts):
+    n = sum(counts)
+    pos, vel = f4(n), f4(n)
+    pos[:, :3] = rng.uniform(-5, 5, (n, 3))
+    vel[:, :3] = rng.normal(0, 0.1, (n, 3))
+    tags = np.concatenate([np.full(c, float(t), np.float32) for t, c in enumerate(counts)])
+    pos[:, 3] = tags
+    vel[:, 3] = np.where((tags == 0) | (tags == 2), 2.364458, 1.0)
+    return pos, vel
+
+
+def test_impact_initialize_reorders_offsets_and_centres(oracle):
+    """impactInitialization's set-up: with zero offsets the frame is the
+    generator's state permuted [Fe1, Si1, Fe2, Si2] -> [Fe1, Fe2, Si1, Si2]
+    (tags 0, 2, 1, 3 in that order); offsets move each body by exactly the
+    float of offset / UnitLength (velocities by offset * UnitTime /
+    UnitLength); the stats are the mass-weighted centre, mean velocity and
+    angular momentum about them (checked with an independent cross-product
+    form)."""
+    rng = np.random.default_rng(31)
+    counts = (7, 20, 5, 13)  # Fe1 Si1 Fe2 Si2
+    g_pos, g_vel = _generator_state(rng, counts)
+    z = np.zeros(3, np.float32)
+    pos, vel, st = oracle.impact_initialize(g_pos, g_vel, *counts, z, z, z, z, 100.0, 3000.0, 2.364458)
+    order = np.concatenate([np.arange(0, 7), np.arange(27, 32), np.arange(7, 27), np.arange(32, 45)])
+    assert np.array_equal(pos, g_pos[order]) and np.array_equal(vel, g_vel[order])
+    assert list(pos[:, 3]) == [0.0] * 7 + [2.0] * 5 + [1.0] * 20 + [3.0] * 13
+    ip1 = np.array([-250.0, 10.0, 3.5], np.float32)
+    ip2 = np.array([400.0, -20.0, 0.0], np.float32)
+    iv1 = np.array([1.5, 0.0, -0.25], np.float32)
+    iv2 = np.array([-3.0, 0.5, 0.0], np.float32)
+    ul, ut = 1161.65, 2935.86
+    pos, vel, st = oracle.impact_initialize(g_pos, g_vel, *counts, ip1, ip2, iv1, iv2, ul, ut, 2.364458)
+    body1 = np.isin(pos[:, 3], [0.0, 1.0])
+    d1 = (ip1.astype(np.float64) / ul).astype(np.float32)
+    d2 = (ip2.astype(np.float64) / ul).astype(np.float32)
+    u1 = (iv1.astype(np.float64) * (ut / ul)).astype(np.float32)
+    u2 = (iv2.astype(np.float64) * (ut / ul)).astype(np.float32)
+    base_p, base_v = g_pos[order][:, :3], g_vel[order][:, :3]
+    assert np.array_equal(pos[body1, :3], base_p[body1] + d1) and np.array_equal(pos[~body1, :3], base_p[~body1] + d2)
+    assert np.array_equal(vel[body1, :3], base_v[body1] + u1) and np.array_equal(vel[~body1, :3], base_v[~body1] + u2)
+    nfe = 12
+    m = np.where(np.arange(45) < nfe, 2.364458, 1.0)
+    X, V = pos[:, :3].astype(np.float64), vel[:, :3].astype(np.float64)
+    c = (m[:, None] * X).sum(0) / m.sum()
+    u = (m[:, None] * V).sum(0) / m.sum()
+    L = (m[:, None] * np.cross(X, V)).sum(0) - m.sum() * np.cross(c, u)
+    assert st[0] == pytest.approx(m.sum(), rel=1e-15)
+    assert np.allclose(st[1:4], c, rtol=1e-13, atol=1e-13)
+    assert np.allclose(st[4:7], u, rtol=1e-13, atol=1e-13)
+    assert np.allclose(st[7:10], L, rtol=1e-10, atol=1e-10 * np.abs(L).max())
