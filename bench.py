@@ -119,9 +119,11 @@ def ensemble_leg(gi, n_sims: int, steps: int, seq_steps: int = 100, check_every:
     c0 = ens.conserved()
     ens.steps(1)
     gi.sync()
-    # batched: `steps` steps in segments of check_every, FP64 E / L of every
-    # simulation between segments (outside the timed region)
-    t_batched = 0.0
+    # batched: `steps` steps in segments of check_every with every
+    # simulation's FP64 E / L after each segment, all inside the timed region
+    # (the workload asks for those reductions every 100 steps); the steps
+    # alone are timed too
+    t_batched = t_steps_only = 0.0
     done = 0
     dE_series, dL_series = [], []
     scaleL = np.maximum(np.linalg.norm(c0[:, 6:9], axis=1), 1e-30)
@@ -130,9 +132,11 @@ def ensemble_leg(gi, n_sims: int, steps: int, seq_steps: int = 100, check_every:
         t0 = time.perf_counter()
         ens.steps(k)
         gi.sync()
-        t_batched += time.perf_counter() - t0
-        done += k
+        t1 = time.perf_counter()
         c = ens.conserved()
+        t_batched += time.perf_counter() - t0
+        t_steps_only += t1 - t0
+        done += k
         dE_series.append(float((np.abs(c[:, 2] - c0[:, 2]) / np.abs(c0[:, 2])).max()))
         dL_series.append(float((np.linalg.norm(c[:, 6:9] - c0[:, 6:9], axis=1) / scaleL).max()))
     c1 = c
@@ -180,6 +184,7 @@ def ensemble_leg(gi, n_sims: int, steps: int, seq_steps: int = 100, check_every:
         "steps": steps,
         "sequential_steps": seq_steps,
         "batched_sim_steps_per_s": batched_rate,
+        "batched_sim_steps_per_s_steps_only": n_sims * steps / t_steps_only,
         "sequential_plain_kernel_sim_steps_per_s": plain_rate,
         "speedup_vs_sequential": batched_rate / plain_rate,
         "sequential_graph_stepper_sim_steps_per_s": seq_rate,
@@ -193,13 +198,12 @@ def ensemble_leg(gi, n_sims: int, steps: int, seq_steps: int = 100, check_every:
     }
 
 
-def serving_leg(gi, n_requests: int, mean_gap_s: float, seed: int = 20261206) -> dict:
-    """The request loop under Poisson arrivals: N log-uniform in [2048, 65536]
-    (impactor fraction [0.1, 0.3]), steps uniform in [1000, 20000], impact
-    angle [30, 60] deg, speed [1.0, 1.4] v_esc, target spin [0, 0.6] breakup;
-    exponential inter-arrival gaps of mean mean_gap_s.  Reports time to result
-    (p50 / p99), aggregate throughput and the per-request FP64 energy and
-    angular-momentum changes."""
+def serve_requests(n_requests: int, mean_gap_s: float, seed: int = 20261206) -> list:
+    """The request stream: N log-uniform in [2048, 65536] (impactor fraction
+    [0.1, 0.3]), steps uniform in [1000, 20000], impact angle [30, 60] deg,
+    speed [1.0, 1.4] v_esc, target spin [0, 0.6] breakup; exponential
+    inter-arrival gaps of mean mean_gap_s (the same requests for every gap:
+    the arrival times are the unit gaps scaled)."""
     rng = np.random.default_rng(seed)
     t = 0.0
     reqs = []
@@ -209,10 +213,42 @@ def serving_leg(gi, n_requests: int, mean_gap_s: float, seed: int = 20261206) ->
         reqs.append(dict(n_target=n - ni, n_impactor=ni, seed=seed + 17 * i, angle_deg=float(rng.uniform(30, 60)),
                          speed_vesc=float(rng.uniform(1.0, 1.4)), target_spin=float(rng.uniform(0.0, 0.6)),
                          steps=int(rng.integers(1000, 20001)), arrival_s=t))
-        t += float(rng.exponential(mean_gap_s))
+        t += float(rng.exponential(1.0)) * mean_gap_s
+    return reqs
+
+
+def engine_on_mix(gi, reqs: list, steps: int = 40) -> dict:
+    """The batched engine on the stream's requests all resident at once (one
+    ensemble, `steps` timed steps after 2 untimed): the rate continuous
+    batching is compared with."""
+    cfg = [{k: q[k] for k in ("n_target", "n_impactor", "seed", "angle_deg", "speed_vesc", "target_spin")}
+           for q in reqs]
+    pos, vel, n, fc, mc, _ = gi.make_ensemble(cfg)
+    P, V = gi.DeviceArray(len(pos), pos), gi.DeviceArray(len(pos), vel)
+    ens = gi.Ensemble(P, V, n, fc, mc)
+    ens.steps(2)
+    gi.sync()
+    t0 = time.perf_counter()
+    ens.steps(steps)
+    gi.sync()
+    dt = time.perf_counter() - t0
+    ens.close()
+    pairs = float((n.astype(np.float64) ** 2).sum())
+    return {"sims": len(n), "steps": steps, "sim_steps_per_s": len(n) * steps / dt, "pair_per_s": pairs * steps / dt}
+
+
+def serving_leg(gi, n_requests: int, mean_gap_s: float, engine: dict, seed: int = 20261206) -> dict:
+    """The request loop under Poisson arrivals.  Reports time to result
+    (p50 / p99), aggregate throughput against the batched engine's on the
+    same requests all resident (pair interactions per second: the requests'
+    sizes weigh the same in both), and the per-request FP64 energy and
+    angular-momentum changes."""
+    reqs = serve_requests(n_requests, mean_gap_s, seed)
     res, summ = gi.serve(reqs)
     ns = np.array([r["n"] for r in res], np.float64)
     st = np.array([r["steps"] for r in res], np.float64)
+    dl = np.array([r["angmom_rel_change"] for r in res])
+    worst = int(np.argmax(dl))
     return {
         "requests": n_requests,
         "arrivals": f"Poisson, mean gap {mean_gap_s} s (last arrival {reqs[-1]['arrival_s']:.1f} s)",
@@ -222,13 +258,17 @@ def serving_leg(gi, n_requests: int, mean_gap_s: float, seed: int = 20261206) ->
         "wall_s": summ["wall_s"],
         "sim_steps_per_s": summ["sim_steps_per_s"],
         "pair_per_s": summ["pair_interactions_per_s"],
+        "fraction_of_engine_pair_rate": summ["pair_interactions_per_s"] / engine["pair_per_s"],
         "p50_time_to_result_s": summ["p50_time_to_result_s"],
         "p99_time_to_result_s": summ["p99_time_to_result_s"],
         "max_time_to_result_s": summ["max_time_to_result_s"],
         "peak_resident_sims": summ["peak_resident_sims"],
         "peak_resident_particles": summ["peak_resident_particles"],
+        "rounds": summ["rounds"],
         "fp64_energy_rel_change_max": summ["max_energy_rel_change"],
         "fp64_angmom_rel_change_max": summ["max_angmom_rel_change"],
+        "fp64_angmom_worst_request": {"index": worst, "n": int(ns[worst]), "steps": int(st[worst]),
+                                      "angmom_rel_change": float(dl[worst])},
     }
 
 
@@ -416,6 +456,9 @@ def main() -> int:
     ap.add_argument("--serve-requests", type=int, default=24,
                     help="requests of the serving leg (0 = skip)")
     ap.add_argument("--serve-gap", type=float, default=1.5, help="mean Poisson inter-arrival gap (s)")
+    ap.add_argument("--serve-gap-saturated", type=float, default=0.25,
+                    help="mean gap of a second run of the same requests arriving faster than they are served "
+                         "(0 = skip)")
     ap.add_argument("--drift-steps", type=int, default=10000,
                     help="steps of the N=65,536 conserved-quantity run (0 = skip the drift leg)")
     ap.add_argument("--drift-headline-steps", type=int, default=2000,
@@ -594,9 +637,16 @@ def main() -> int:
         result["ensemble"] = ens
         log(f"ensemble: {result['ensemble']}")
     if args.serve_requests > 0:
+        engine = engine_on_mix(gi, serve_requests(args.serve_requests, args.serve_gap))
+        log(f"batched engine on the serving mix: {engine}")
         log(f"serving leg: {args.serve_requests} requests, Poisson gaps of mean {args.serve_gap} s")
-        result["serving"] = serving_leg(gi, args.serve_requests, args.serve_gap)
+        result["serving"] = serving_leg(gi, args.serve_requests, args.serve_gap, engine)
+        result["serving"]["engine_on_mix"] = engine
         log(f"serving: {result['serving']}")
+        if args.serve_gap_saturated > 0:
+            log(f"serving leg, saturated: the same requests, gaps of mean {args.serve_gap_saturated} s")
+            result["serving_saturated"] = serving_leg(gi, args.serve_requests, args.serve_gap_saturated, engine)
+            log(f"serving (saturated): {result['serving_saturated']}")
     if args.drift_steps > 0:
         log(f"drift leg: N=2048 x 1000 steps against the oracle, N=65536 x {args.drift_steps} steps, "
             f"N=262144 x {args.drift_headline_steps} steps")

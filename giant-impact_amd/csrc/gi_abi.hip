@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "../../include/giant_impact.h"
+#include "gi_graph_cache.h"
 #include "gi_tuning.h"
 
 static_assert(sizeof(gi_float4) == 16, "float4 layout");
@@ -145,10 +146,8 @@ struct Workspace {
     unsigned long long generation = 0;  // bumped on every re-plan
 };
 
-// A captured step sequence and what it was captured for.
-struct GraphEntry {
-    std::vector<unsigned char> key;
-    unsigned long long generation = 0;
+// A captured step sequence (the graph cache's payload).
+struct GraphPayload {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     void destroy() {
@@ -158,6 +157,7 @@ struct GraphEntry {
         graph = nullptr;
     }
 };
+using GraphCacheHip = GraphCache<GraphPayload>;
 
 // Scratch of the Earth / Moon finder.
 struct CCWork {
@@ -182,7 +182,7 @@ struct DevCtx {
     // into the direct pass (steps alternate between the caller's arrays and
     // these)
     DevBuf<float4> alt_pos, alt_vel;
-    std::vector<GraphEntry> graphs;
+    GraphCacheHip graphs;
     std::vector<TimingPair> pending, pool;
     long long timed_launches[kTimedKinds] = {0, 0, 0, 0};
     double timed_ms[kTimedKinds] = {0.0, 0.0, 0.0, 0.0};
@@ -930,58 +930,82 @@ static void key_append(std::vector<unsigned char>& k, const T& v) {
     k.insert(k.end(), b, b + sizeof(T));
 }
 
-// Capture `body` on a private stream into a graph cached under `key`.  The
-// cache keeps the kGraphCacheSize most recently used graphs; `keep` lists
-// graphs the caller already holds, which are never evicted (a caller that
-// needs several graphs at once asks for them one by one).  An evicted graph
-// may still have launches in flight from an earlier call on any stream, so
-// the device is synchronised before it is destroyed.
+// Capture `body` on a private stream into a graph cached under `key` (the
+// lookup and eviction policy is GraphCache's: the kGraphCacheSize most
+// recently used graphs are kept; `keep` lists graphs the caller already holds,
+// which are never evicted; the device is synchronised before an evicted graph
+// is destroyed, since it may still have launches in flight from an earlier
+// call on any stream).
 constexpr size_t kGraphCacheSize = 8;
 
+// With update, a graph of the same key captured for an earlier plan
+// generation is updated in place (hipGraphExecUpdate: the executable graph
+// stays, its nodes take the new plan's parameters) when the new capture has
+// the same topology; otherwise a new one is instantiated.
 template <typename F>
-static int get_graph(DevCtx* ctx, std::vector<GraphEntry>& cache, const std::vector<unsigned char>& key,
+static int get_graph(DevCtx* ctx, GraphCacheHip& cache, const std::vector<unsigned char>& key,
                      unsigned long long generation, F body, hipGraphExec_t* out,
-                     std::initializer_list<hipGraphExec_t> keep = {}) {
-    for (size_t k = 0; k < cache.size(); ++k)
-        if (cache[k].key == key && cache[k].generation == generation) {
-            GraphEntry hit = cache[k];  // most recently used goes last
-            cache.erase(cache.begin() + (long)k);
-            cache.push_back(hit);
-            *out = hit.exec;
-            return 0;
+                     std::initializer_list<hipGraphExec_t> keep = {}, bool update = false) {
+    cache.capacity = kGraphCacheSize;
+    auto record = [&](hipGraph_t* g) -> int {
+        if (!ctx->capture_stream) GI_HIP(hipStreamCreateWithFlags(&ctx->capture_stream, hipStreamNonBlocking));
+        GI_HIP(hipStreamBeginCapture(ctx->capture_stream, hipStreamCaptureModeThreadLocal));
+        int rc = body(ctx->capture_stream);
+        *g = nullptr;
+        hipError_t ce = hipStreamEndCapture(ctx->capture_stream, g);
+        if (rc) {
+            if (*g) (void)hipGraphDestroy(*g);
+            *g = nullptr;
+            return rc;
         }
-    if (!ctx->capture_stream) GI_HIP(hipStreamCreateWithFlags(&ctx->capture_stream, hipStreamNonBlocking));
-    GraphEntry e;
-    e.key = key;
-    e.generation = generation;
-    GI_HIP(hipStreamBeginCapture(ctx->capture_stream, hipStreamCaptureModeThreadLocal));
-    int rc = body(ctx->capture_stream);
-    hipGraph_t g = nullptr;
-    hipError_t ce = hipStreamEndCapture(ctx->capture_stream, &g);
-    if (rc) {
-        if (g) (void)hipGraphDestroy(g);
-        return rc;
-    }
-    if (ce != hipSuccess) return set_error((int)ce, "hipStreamEndCapture: %s", hipGetErrorString(ce));
-    e.graph = g;
-    hipError_t ie = hipGraphInstantiate(&e.exec, g, nullptr, nullptr, 0);
-    if (ie != hipSuccess) {
-        (void)hipGraphDestroy(g);
-        return set_error((int)ie, "hipGraphInstantiate: %s", hipGetErrorString(ie));
-    }
-    if (cache.size() >= kGraphCacheSize) {
-        for (size_t k = 0; k < cache.size(); ++k) {
-            bool held = false;
-            for (hipGraphExec_t h : keep) held = held || cache[k].exec == h;
-            if (held) continue;
-            GI_HIP(hipDeviceSynchronize());
-            cache[k].destroy();
-            cache.erase(cache.begin() + (long)k);
-            break;
+        if (ce != hipSuccess) return set_error((int)ce, "hipStreamEndCapture: %s", hipGetErrorString(ce));
+        return 0;
+    };
+    auto refresh = [&](GraphPayload& p, bool* done) -> int {
+        *done = false;
+        if (!update) return 0;
+        hipGraph_t g = nullptr;
+        int rc = record(&g);
+        if (rc) return rc;
+        hipGraphNode_t bad = nullptr;
+        hipGraphExecUpdateResult res = hipGraphExecUpdateError;
+        if (hipGraphExecUpdate(p.exec, g, &bad, &res) == hipSuccess && res == hipGraphExecUpdateSuccess) {
+            if (p.graph) (void)hipGraphDestroy(p.graph);
+            p.graph = g;
+            *done = true;
+        } else {
+            (void)hipGraphDestroy(g);
+            (void)hipGetLastError();  // a refused update is not an error of the call
         }
-    }
-    cache.push_back(e);
-    *out = e.exec;
+        return 0;
+    };
+    auto capture = [&](GraphPayload* e) -> int {
+        hipGraph_t g = nullptr;
+        int rc = record(&g);
+        if (rc) return rc;
+        e->graph = g;
+        hipError_t ie = hipGraphInstantiate(&e->exec, g, nullptr, nullptr, 0);
+        if (ie != hipSuccess) {
+            (void)hipGraphDestroy(g);
+            e->graph = nullptr;
+            return set_error((int)ie, "hipGraphInstantiate: %s", hipGetErrorString(ie));
+        }
+        return 0;
+    };
+    auto sync = [&]() -> int {
+        GI_HIP(hipDeviceSynchronize());
+        return 0;
+    };
+    auto destroy = [](GraphPayload& p) { p.destroy(); };
+    auto held = [&](const GraphPayload& p) {
+        for (hipGraphExec_t h : keep)
+            if (p.exec == h) return true;
+        return false;
+    };
+    GraphPayload got;
+    int rc = cache.get(key, generation, capture, refresh, sync, destroy, held, &got);
+    if (rc) return rc;
+    *out = got.exec;
     return 0;
 }
 
@@ -1052,18 +1076,29 @@ struct gi_ensemble {
     int device = -1;
     gi::Workspace ws;       // the stepping plan (active simulations)
     gi::Workspace ws_diag;  // all simulations, for the conserved quantities
+    // The simulation table over [0, end of the last simulation): the caller's
+    // simulations in offset order, with frozen fillers over the gaps between
+    // them (gi_ensemble_set_sims), so that the table covers its particles
+    // back to back.  caller[k] = the caller's index of sims[k], -1 for a filler.
     std::vector<gi::SimDesc> sims;
+    std::vector<int> caller;
+    int n_caller = 0;
+    long long capacity = 0;  // records in pos / vel
     float4* pos = nullptr;
     float4* vel = nullptr;
     gi::DevBuf<float4> force;
     hipStream_t stream = nullptr;
-    std::vector<gi::GraphEntry> graphs;
+    gi::GraphCacheHip graphs;
+    // one event per gi_ensemble_steps call still possibly running (oldest first)
+    std::vector<hipEvent_t> pending, spare;
     ~gi_ensemble() {
         // launches of its graphs may still be in flight on its device
         int cur = -1;
         if (hipGetDevice(&cur) == hipSuccess && device >= 0 && cur != device) (void)hipSetDevice(device);
         (void)hipDeviceSynchronize();
-        for (auto& g : graphs) g.destroy();
+        graphs.clear([](gi::GraphPayload& p) { p.destroy(); });
+        for (hipEvent_t ev : pending) (void)hipEventDestroy(ev);
+        for (hipEvent_t ev : spare) (void)hipEventDestroy(ev);
         if (cur >= 0 && cur != device) (void)hipSetDevice(cur);
     }
 };
@@ -1558,6 +1593,9 @@ static int impact_steps_impl(gi_float4* pos, gi_float4* vel, gi_float4* force, i
         key_append(k2, V);
         key_append(k2, n);
         key_append(k2, fc->NFe);
+        // The recenter's scratch exists before capture, and its address is
+        // part of the key: a graph holding a buffer that was since
+        // reallocated is never launched again.
         if (rs.ref_float) {
             key_append(k2, rs.mass_fe);
             key_append(k2, rs.mass_si);
@@ -1565,7 +1603,8 @@ static int impact_steps_impl(gi_float4* pos, gi_float4* vel, gi_float4* force, i
                 key_append(k2, rs.tcf[a]);
                 key_append(k2, rs.tvf[a]);
             }
-            if ((rc = ctx->rc_shift.ensure(8))) return rc;  // scratch exists before capture
+            if ((rc = ctx->rc_shift.ensure(8))) return rc;
+            key_append(k2, ctx->rc_shift.p);
             rc = get_graph(ctx, ctx->graphs, k2, ctx->ws.generation,
                            [&](hipStream_t cs) -> int {
                                return launch_recenter_ref(ctx, P, V, n, fc->NFe, rs.mass_fe, rs.mass_si, rs.tcf,
@@ -1580,9 +1619,10 @@ static int impact_steps_impl(gi_float4* pos, gi_float4* vel, gi_float4* force, i
                 key_append(k2, rs.tcd[a]);
                 key_append(k2, rs.tvd[a]);
             }
-            // make sure the recenter scratch exists before capture
             if ((rc = ctx->rc_blocks.ensure((size_t)nblk(n, 256) * kMoments1 + 1))) return rc;
             if ((rc = ctx->rc_sums.ensure(kMoments1 + 1))) return rc;
+            key_append(k2, ctx->rc_blocks.p);
+            key_append(k2, ctx->rc_sums.p);
             rc = get_graph(ctx, ctx->graphs, k2, ctx->ws.generation,
                            [&](hipStream_t cs) -> int {
                                return launch_recenter(ctx, P, V, n, fc->NFe, mfe, msi, rs.tcd, rs.tvd, cs);
@@ -1739,8 +1779,11 @@ int gi_ensemble_create(gi_ensemble** out, int n_sims, const int* n, const gi_for
         d.dt_over_mass_fe = mc[s].DtOverMassFe;
         d.dt_over_mass_si = mc[s].DtOverMassSi;
         e->sims.push_back(d);
+        e->caller.push_back(s);
         off += n[s];
     }
+    e->n_caller = n_sims;
+    e->capacity = off;
     e->pos = (float4*)pos;
     e->vel = (float4*)vel;
     e->stream = as_stream(stream);
@@ -1754,6 +1797,66 @@ int gi_ensemble_create(gi_ensemble** out, int n_sims, const int* n, const gi_for
     return 0;
 }
 
+int gi_ensemble_create_capacity(gi_ensemble** out, int capacity, gi_float4* pos, gi_float4* vel,
+                                gi_stream_t stream) {
+    if (!out || capacity <= 0 || !pos || !vel)
+        return set_error(GI_E_INVALID_ARG, "gi_ensemble_create_capacity: invalid arguments");
+    if (capacity > (1 << 21)) return set_error(GI_E_TOO_LARGE, "gi_ensemble_create_capacity: more than 2^21 particles");
+    *out = nullptr;
+    int dev = 0;
+    GI_HIP(hipGetDevice(&dev));
+    gi_ensemble* e = new gi_ensemble();
+    e->device = dev;
+    e->ws.batched = true;
+    e->ws_diag.batched = true;
+    e->capacity = capacity;
+    e->pos = (float4*)pos;
+    e->vel = (float4*)vel;
+    e->stream = as_stream(stream);
+    int rc = e->force.ensure((size_t)capacity);
+    if (rc) {
+        delete e;
+        return rc;
+    }
+    *out = e;
+    return 0;
+}
+
+int gi_ensemble_set_sims(gi_ensemble* e, int n_sims, const int* offset, const int* n,
+                         const gi_force_collision_constants* fc, const gi_move_collision_constants* mc) {
+    if (!e || n_sims < 0 || (n_sims > 0 && (!offset || !n || !fc || !mc)))
+        return set_error(GI_E_INVALID_ARG, "gi_ensemble_set_sims: invalid arguments");
+    std::vector<SimDesc> sims;
+    std::vector<int> caller;
+    long long cursor = 0;
+    for (int s = 0; s < n_sims; ++s) {
+        if (n[s] <= 0 || offset[s] < cursor || (long long)offset[s] + n[s] > e->capacity || fc[s].NFe < 0 ||
+            fc[s].NFe > n[s] || mc[s].NFe != fc[s].NFe)
+            return set_error(GI_E_INVALID_ARG,
+                             "gi_ensemble_set_sims: simulation %d must have n > 0, NFe in [0, n], and lie after the "
+                             "previous one inside the capacity",
+                             s);
+        if (offset[s] > cursor) {
+            // a frozen filler over the gap: never stepped, its records never written
+            SimDesc f = make_sim((int)cursor, (int)cursor, offset[s], PairLaw{});
+            f.active = 0;
+            sims.push_back(f);
+            caller.push_back(-1);
+        }
+        SimDesc d = make_sim(offset[s], offset[s] + fc[s].NFe, offset[s] + n[s], law_of(&fc[s]));
+        d.dt = mc[s].Dt;
+        d.dt_over_mass_fe = mc[s].DtOverMassFe;
+        d.dt_over_mass_si = mc[s].DtOverMassSi;
+        sims.push_back(d);
+        caller.push_back(s);
+        cursor = (long long)offset[s] + n[s];
+    }
+    e->sims = std::move(sims);
+    e->caller = std::move(caller);
+    e->n_caller = n_sims;
+    return 0;
+}
+
 int gi_ensemble_destroy(gi_ensemble* e) {
     delete e;
     return 0;
@@ -1761,7 +1864,19 @@ int gi_ensemble_destroy(gi_ensemble* e) {
 
 int gi_ensemble_set_active(gi_ensemble* e, const int* active) {
     if (!e || !active) return set_error(GI_E_INVALID_ARG, "gi_ensemble_set_active: invalid arguments");
-    for (size_t s = 0; s < e->sims.size(); ++s) e->sims[s].active = active[s] ? 1 : 0;
+    for (size_t s = 0; s < e->sims.size(); ++s)
+        if (e->caller[s] >= 0) e->sims[s].active = active[e->caller[s]] ? 1 : 0;
+    return 0;
+}
+
+int gi_ensemble_wait(gi_ensemble* e, int max_pending) {
+    if (!e || max_pending < 0) return set_error(GI_E_INVALID_ARG, "gi_ensemble_wait: invalid arguments");
+    while ((int)e->pending.size() > max_pending) {
+        hipEvent_t ev = e->pending.front();
+        GI_HIP(hipEventSynchronize(ev));
+        e->pending.erase(e->pending.begin());
+        e->spare.push_back(ev);
+    }
     return 0;
 }
 
@@ -1772,9 +1887,15 @@ int gi_ensemble_steps(gi_ensemble* e, int n_steps) {
     int rc = current_ctx(&ctx);
     if (rc) return rc;
     if (ctx->device != e->device) return set_error(GI_E_STATE, "gi_ensemble_steps: wrong current device");
+    bool any = false;
+    for (const SimDesc& sd : e->sims) any = any || (sd.active && sd.end > sd.start);
+    if (!any) return 0;
     StreamBridge br;
     if ((rc = br.begin(ctx, e->stream))) return rc;
     const unsigned long long gen0 = e->ws.generation;
+    // A new simulation table may reallocate scratch that earlier launches on
+    // this stream still use: let them finish first.
+    if (!e->ws.plan_valid || !same_sims(e->ws.plan.caller_sims, e->sims)) GI_HIP(hipStreamSynchronize(br.run));
     if ((rc = ensure_plan(&e->ws, e->sims, br.run))) return rc;
     if (e->ws.generation != gen0) GI_HIP(hipStreamSynchronize(br.run));
     const int ntot = e->ws.plan.ntot;
@@ -1794,9 +1915,18 @@ int gi_ensemble_steps(gi_ensemble* e, int n_steps) {
                        GI_HIP(hipGetLastError());
                        return 0;
                    },
-                   &step);
+                   &step, {}, true);
     if (rc) return rc;
     for (int s = 0; s < n_steps; ++s) GI_HIP(hipGraphLaunch(step, br.run));
+    hipEvent_t ev;
+    if (!e->spare.empty()) {
+        ev = e->spare.back();
+        e->spare.pop_back();
+    } else {
+        GI_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    }
+    GI_HIP(hipEventRecord(ev, br.run));
+    e->pending.push_back(ev);
     return br.end();
 }
 
@@ -1806,7 +1936,8 @@ int gi_ensemble_conserved(gi_ensemble* e, double* out) {
     int rc = current_ctx(&ctx);
     if (rc) return rc;
     std::vector<SimDesc> all = e->sims;
-    for (auto& s : all) s.active = 1;
+    for (size_t s = 0; s < all.size(); ++s) all[s].active = e->caller[s] >= 0 ? 1 : 0;  // fillers stay frozen
+    if (all.empty()) return 0;
     StreamBridge br;
     if ((rc = br.begin(ctx, e->stream))) return rc;
     if ((rc = ensure_plan(&e->ws_diag, all, br.run))) return rc;
@@ -1815,7 +1946,8 @@ int gi_ensemble_conserved(gi_ensemble* e, double* out) {
     std::vector<double> m(ns * kMoments);
     GI_HIP(hipMemcpyAsync(m.data(), e->ws_diag.sums.p, m.size() * sizeof(double), hipMemcpyDeviceToHost, br.run));
     GI_HIP(hipStreamSynchronize(br.run));
-    for (size_t s = 0; s < ns; ++s) moments_to_conserved(&m[s * kMoments], out + 9 * s);
+    for (size_t s = 0; s < ns; ++s)
+        if (e->caller[s] >= 0) moments_to_conserved(&m[s * kMoments], out + 9 * e->caller[s]);
     return br.end();
 }
 

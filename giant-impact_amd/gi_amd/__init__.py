@@ -269,6 +269,9 @@ def load() -> C.CDLL:
             [vp, vp, vp, i, P(ForceCollisionConstants), P(MoveCollisionConstants), i, i, P(C.c_double), P(C.c_double), vp],
         ),
         "gi_ensemble_create": (i, [P(vp), i, P(i), vp, vp, vp, vp, vp]),
+        "gi_ensemble_create_capacity": (i, [P(vp), i, vp, vp, vp]),
+        "gi_ensemble_set_sims": (i, [vp, i, P(i), P(i), vp, vp]),
+        "gi_ensemble_wait": (i, [vp, i]),
         "gi_ensemble_destroy": (i, [vp]),
         "gi_ensemble_steps": (i, [vp, i]),
         "gi_ensemble_set_active": (i, [vp, P(i)]),
@@ -553,6 +556,29 @@ class Ensemble:
             V.ptr,
             _p(stream),
         )
+
+    @classmethod
+    def with_capacity(cls, P: DeviceArray, V: DeviceArray, capacity: int, stream=None) -> "Ensemble":
+        """gi_ensemble_create_capacity: a pool of `capacity` records with no
+        simulations yet (place them with set_sims)."""
+        self = cls.__new__(cls)
+        self.n = np.zeros(0, np.int32)
+        self.n_sims = 0
+        self.offsets = np.zeros(1, np.int64)
+        self.handle = C.c_void_p()
+        call("gi_ensemble_create_capacity", C.byref(self.handle), int(capacity), P.ptr, V.ptr, _p(stream))
+        return self
+
+    def set_sims(self, offsets, n, fc, mc) -> None:
+        """gi_ensemble_set_sims: simulation s at records [offsets[s], offsets[s] + n[s])."""
+        o = np.ascontiguousarray(offsets, dtype=np.int32)
+        self.n = np.ascontiguousarray(n, dtype=np.int32)
+        self.n_sims = len(self.n)
+        call("gi_ensemble_set_sims", self.handle, self.n_sims, o.ctypes.data_as(C.POINTER(C.c_int)),
+             self.n.ctypes.data_as(C.POINTER(C.c_int)), C.cast(fc, C.c_void_p), C.cast(mc, C.c_void_p))
+
+    def wait(self, max_pending: int = 0) -> None:
+        call("gi_ensemble_wait", self.handle, int(max_pending))
 
     def steps(self, k: int) -> None:
         call("gi_ensemble_steps", self.handle, int(k))

@@ -271,6 +271,27 @@ int gi_ensemble_create(gi_ensemble **out, int n_sims, const int *n,
                        const gi_force_collision_constants *fc,
                        const gi_move_collision_constants *mc, gi_float4 *pos, gi_float4 *vel,
                        gi_stream_t stream);
+/* An ensemble over caller-owned arrays of `capacity` records with no
+ * simulations yet: gi_ensemble_set_sims places them.  For a resident pool
+ * whose simulations come and go (the request loop).  No reference
+ * counterpart. */
+int gi_ensemble_create_capacity(gi_ensemble **out, int capacity, gi_float4 *pos, gi_float4 *vel,
+                                gi_stream_t stream);
+/* Replace the simulation table: simulation s occupies records
+ * [offset[s], offset[s] + n[s]) of the ensemble's arrays (offsets ascending,
+ * no overlap, inside the capacity), iron first (fc[s].NFe irons).  Gaps
+ * between simulations are left untouched.  Every simulation starts active.
+ * Only the table changes: the arrays stay where they are, state in them is
+ * not moved, the scratch is kept (grown when needed) and the captured step
+ * graph is updated in place at the next gi_ensemble_steps.  Simulation
+ * indices of the other calls (set_active, conserved) follow this table.
+ * No reference counterpart. */
+int gi_ensemble_set_sims(gi_ensemble *e, int n_sims, const int *offset, const int *n,
+                         const gi_force_collision_constants *fc, const gi_move_collision_constants *mc);
+/* Block until at most max_pending of this ensemble's gi_ensemble_steps calls
+ * may still be running on the device (0: all done).  No reference
+ * counterpart. */
+int gi_ensemble_wait(gi_ensemble *e, int max_pending);
 int gi_ensemble_destroy(gi_ensemble *e);
 /* Advance every simulation whose active flag is set by n_steps steps. */
 int gi_ensemble_steps(gi_ensemble *e, int n_steps);

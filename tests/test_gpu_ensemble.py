@@ -188,3 +188,54 @@ def test_contact_walk_forms_are_bitwise_the_same(gpu):
         p1, v1 = _ensemble_of_one(gpu, pos[a:b], vel[a:b], fc[s], mc[s], 4)
         assert np.array_equal(p_all[a:b], p1), s
         assert np.array_equal(v_all[a:b], v1), s
+
+
+def test_resident_pool_table_changes_are_bitwise_packed_ensembles(gpu):
+    """An ensemble over a pool of records whose simulation table changes
+    (gi_ensemble_set_sims: gaps between simulations, one leaving, one
+    arriving into the freed range) steps each simulation bitwise as an
+    ensemble of one does, and never writes the records between them."""
+    cfg = gpu.ensemble_configs(4, seed=21, n_range=(900, 3500))
+    pos, vel, n, fc, mc, _ = gpu.make_ensemble(cfg)
+    src = np.concatenate([[0], np.cumsum(n)])
+    off = np.array([100, 100 + n[0] + 777, 100 + n[0] + 777 + n[1], 100 + n[0] + 777 + n[1] + n[2] + 50])
+    cap = int(off[-1] + n[3] + 300)
+    rng = np.random.default_rng(3)
+    pool_p = f4(cap)
+    pool_p[:, :3] = rng.uniform(-1e4, 1e4, (cap, 3))
+    pool_v = f4(cap)
+    for s in range(4):
+        pool_p[off[s]:off[s] + n[s]] = pos[src[s]:src[s + 1]]
+        pool_v[off[s]:off[s] + n[s]] = vel[src[s]:src[s + 1]]
+    P, V = gpu.DeviceArray(cap, pool_p), gpu.DeviceArray(cap, pool_v)
+    ens = gpu.Ensemble.with_capacity(P, V, cap)
+    FC, MC = gpu.ForceCollisionConstants, gpu.MoveCollisionConstants
+    ens.set_sims(off, n, (FC * 4)(*fc), (MC * 4)(*mc))
+    ens.steps(5)
+    ens.wait(0)
+    after5 = P.download(), V.download()
+    # simulation 1 leaves; 3 moves on alone with 0 and 2 for 7 more steps
+    keep = [0, 2, 3]
+    ens.set_sims(off[keep], n[keep], (FC * 3)(*[fc[k] for k in keep]), (MC * 3)(*[mc[k] for k in keep]))
+    ens.steps(3)
+    ens.steps(4)
+    ens.wait(1)
+    ens.wait(0)
+    p12, v12 = P.download(), V.download()
+    # a new simulation (a copy of 1's initial state) enters the freed range
+    ens.set_sims(off, n, (FC * 4)(*fc), (MC * 4)(*mc))
+    P.upload(np.concatenate([p12[:off[1]], pos[src[1]:src[2]], p12[off[1] + n[1]:]]))
+    V.upload(np.concatenate([v12[:off[1]], vel[src[1]:src[2]], v12[off[1] + n[1]:]]))
+    ens.steps(2)
+    gpu.sync()
+    p14, v14 = P.download(), V.download()
+    ens.close()
+    for s, k, (pp, vv) in ((0, 12, (p12, v12)), (2, 12, (p12, v12)), (3, 12, (p12, v12)), (1, 5, after5),
+                            (0, 14, (p14, v14)), (1, 2, (p14, v14))):
+        p1, v1 = _ensemble_of_one(gpu, pos[src[s]:src[s + 1]], vel[src[s]:src[s + 1]], fc[s], mc[s], k)
+        sl = slice(off[s], off[s] + n[s])
+        assert np.array_equal(pp[sl], p1) and np.array_equal(vv[sl], v1), (s, k)
+    holes = np.ones(cap, bool)
+    for s in range(4):
+        holes[off[s]:off[s] + n[s]] = False
+    assert np.array_equal(p14[holes], pool_p[holes]) and np.array_equal(v14[holes], pool_v[holes])
