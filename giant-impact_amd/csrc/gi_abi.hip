@@ -1072,6 +1072,39 @@ struct StreamBridge {
 // Ensemble
 // ===========================================================================
 
+namespace gi {
+
+// ---------------------------------------------------------------------------
+// The allocation-free force entry point: its whole scratch (the simulation's
+// descriptor and the direct pass's block table) lives in one file-scope
+// device array, so a call allocates nothing.
+// ---------------------------------------------------------------------------
+
+constexpr int kNoAllocMaxN = 12288;                    // particles per call, at most
+constexpr int kNoAllocMaxBlocks = kNoAllocMaxN / 16;  // >= 16 i-particles per workgroup
+struct NoAllocScratch {
+    SimDesc sim;
+    DirectBlock blocks[kNoAllocMaxBlocks];
+};
+
+// Host code must never pass this name to a kernel (on the host it is a
+// stand-in, not a device address): gi_force_collision_noalloc resolves each
+// device's own address with hipGetSymbolAddress and passes that.
+static __device__ __align__(256) unsigned char scratch[sizeof(NoAllocScratch)];
+
+static std::mutex g_noalloc_mutex;
+static NoAllocScratch* g_noalloc_ptr[64];  // per device, set once
+
+// Writes the descriptor and the block table of one simulation of n
+// particles, ib i-particles per workgroup.
+__global__ void k_noalloc_plan(NoAllocScratch* __restrict__ s, SimDesc sd, int nb, int ib) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b == 0) s->sim = sd;
+    if (b < nb) s->blocks[b] = DirectBlock{0, b * ib};
+}
+
+}  // namespace gi
+
 struct gi_ensemble {
     int device = -1;
     gi::Workspace ws;       // the stepping plan (active simulations)
@@ -1282,6 +1315,54 @@ int gi_force_collision(const gi_float4* pos, const gi_float4* vel, gi_float4* fo
     std::vector<SimDesc> sims{make_sim(0, c->NFe, n, law_of(c))};
     if ((rc = ensure_plan(&ctx->ws, sims, st))) return rc;
     return run_force(ctx, &ctx->ws, (const float4*)pos, (const float4*)vel, (float4*)force, st, g_timing);
+}
+
+int gi_force_collision_noalloc(const gi_float4* pos, const gi_float4* vel, gi_float4* force, int n,
+                               const gi_force_collision_constants* c, gi_stream_t stream) {
+    int rc = check_particles("gi_force_collision_noalloc", pos, vel, force, n);
+    if (rc) return rc;
+    if (!c) return set_error(GI_E_INVALID_ARG, "gi_force_collision_noalloc: null constants");
+    if (c->NFe < 0 || c->NFe > n) return set_error(GI_E_INVALID_ARG, "gi_force_collision_noalloc: NFe out of [0,n]");
+    if (n > kNoAllocMaxN)
+        return set_error(GI_E_TOO_LARGE, "gi_force_collision_noalloc: n = %d exceeds its maximum of %d particles", n,
+                         kNoAllocMaxN);
+    if (n == 0) return 0;
+    int dev = 0;
+    GI_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) return set_error(GI_E_NO_DEVICE, "device id %d out of range", dev);
+    NoAllocScratch* sp;
+    {
+        std::lock_guard<std::mutex> lk(g_noalloc_mutex);
+        if (!g_noalloc_ptr[dev]) {
+            void* p = nullptr;
+            GI_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+            g_noalloc_ptr[dev] = (NoAllocScratch*)p;
+        }
+        sp = g_noalloc_ptr[dev];
+    }
+    // The direct pass with the tuning table's lanes per particle for this n
+    // (the plan gi_force_collision makes where the table picks that pass).
+    const Tiling t = choose_tiling(n, (long long)n * n, 1);
+    const int lanes = t.variant == 3 ? std::min(t.R, 4) : 1;
+    const int ib = 64 / lanes;
+    const int nb = nblk(n, ib);
+    hipStream_t st = as_stream(stream);
+    hipLaunchKernelGGL(k_noalloc_plan, dim3(nblk(nb, 256)), dim3(256), 0, st, sp, make_sim(0, c->NFe, n, law_of(c)),
+                       nb, ib);
+    const dim3 grid(nb), block(64 * kDirectWaves);
+    const float4 *P = (const float4*)pos, *V = (const float4*)vel;
+    float4* F = (float4*)force;
+    if (lanes == 4)
+        hipLaunchKernelGGL((k_direct<false, 4, 0>), grid, block, 0, st, P, V, sp->blocks, &sp->sim, F, nullptr,
+                           FusedMove{});
+    else if (lanes == 2)
+        hipLaunchKernelGGL((k_direct<false, 2, 0>), grid, block, 0, st, P, V, sp->blocks, &sp->sim, F, nullptr,
+                           FusedMove{});
+    else
+        hipLaunchKernelGGL((k_direct<false, 1, 0>), grid, block, 0, st, P, V, sp->blocks, &sp->sim, F, nullptr,
+                           FusedMove{});
+    GI_HIP(hipGetLastError());
+    return 0;
 }
 
 int gi_force_separate(const gi_float4* pos, const gi_float4* vel, gi_float4* force, int n,

@@ -272,6 +272,7 @@ def load() -> C.CDLL:
         "gi_ensemble_create_capacity": (i, [P(vp), i, vp, vp, vp]),
         "gi_ensemble_set_sims": (i, [vp, i, P(i), P(i), vp, vp]),
         "gi_ensemble_wait": (i, [vp, i]),
+        "gi_force_collision_noalloc": (i, [vp, vp, vp, i, P(ForceCollisionConstants), vp]),
         "gi_ensemble_destroy": (i, [vp]),
         "gi_ensemble_steps": (i, [vp, i]),
         "gi_ensemble_set_active": (i, [vp, P(i)]),

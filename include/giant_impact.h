@@ -173,6 +173,19 @@ int gi_plan_info(int *mutual, int *r, long long *pair_tasks, long long *chunk_le
 int gi_force_collision(const gi_float4 *pos, const gi_float4 *vel, gi_float4 *force, int n,
                        const gi_force_collision_constants *c, gi_stream_t stream);
 
+/* No counterpart in the reference.  gi_force_collision for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc): the same force, bitwise, where the
+ * tuning table gives gi_force_collision the direct pass (n <= 12,288), with
+ * the direct pass's scratch (the simulation descriptor and the block table,
+ * ~6 KB) in a static device array that each device holds until the process
+ * ends.  Allocates nothing, host or device.  n > 12,288 returns
+ * GI_E_TOO_LARGE.  Every call on one device shares that array: calls that
+ * overlap on one device (several host threads, or several streams without
+ * ordering between them) race. */
+int gi_force_collision_noalloc(const gi_float4 *pos, const gi_float4 *vel, gi_float4 *force, int n,
+                               const gi_force_collision_constants *c, gi_stream_t stream);
+
 /* moveBodiesCollisionSingleGPU (impactAddTime.cu:379-394):
  * vel += force * (Dt/m); pos += vel * Dt  (in that order; .w untouched). */
 int gi_move_collision(gi_float4 *pos, gi_float4 *vel, const gi_float4 *force, int n,
