@@ -103,14 +103,17 @@ struct Plan {
     int ntot = 0;
     int partial_len = 0;  // float4 elements
     int buckets = 0;      // contact-grid buckets over all simulations
-    // mutual all-pairs pass (tiling.variant == 2)
+    // mutual all-pairs pass (tiling.variant == 2, or 4 with row groups)
     bool mutual = false;
+    int group_rows = 1;                // W: rows per group (4 with row groups, else 1)
+    std::vector<GroupTask> gtasks;     // row groups: one per workgroup
     std::vector<MutualTask> mtasks;
     std::vector<RowDesc> rows;
     std::vector<SumBlock> sblocks;  // k_mutual_sum's workgroups
     long long gtot = 0;       // padded particles
     long long iside_len = 0;  // floats
     long long jside_len = 0;  // floats
+    long long jintra_len = 0;  // floats (row groups: pairs inside a group)
     long long chunk_len = 0;  // j per task of the mutual pass (0 if none)
     // every simulation takes the mutual pass: the contact corrections run
     // beside it (k_contact_corr) and k_mutual_sum writes the force
@@ -129,13 +132,15 @@ struct Workspace {
     DevBuf<float4> partial;
     DevBuf<float4> gpos, gsum;     // mutual pass: padded positions, per-particle sums
     DevBuf<float> iside, jside;    // mutual pass: row slots, j-side slabs
+    DevBuf<float> jintra;          // row groups: j-side entries inside a group
+    DevBuf<GroupTask> gtasks;
     DevBuf<MutualTask> mtasks;
     DevBuf<RowDesc> rows;
     DevBuf<SumBlock> sblocks;
     DevBuf<DirectBlock> dblocks;
     DevBuf<int> counts, starts, sorted, cellhash, block_sums;
     DevBuf<float4> spos, svel;  // positions / velocities in bucket order
-    DevBuf<double> corr;        // contact corrections (3 planes of ntot)
+    DevBuf<float> corr;         // contact corrections ((sum, compensation) x 3 planes of ntot)
     DevBuf<GravTask> tasks;
     DevBuf<SimDesc> sims;
     DevBuf<int> block_sim, block_first, sim_first_block;
@@ -250,11 +255,12 @@ static long long sim_chunk_len(long long n) {
     return (L + 63) / 64 * 64;
 }
 
-// j-side slabs of the mutual pass grow as 3 N^2 / (128 R) floats per
-// simulation; a simulation whose slab would exceed this many floats (16 GiB:
-// about N > 1.2 M at R = 8, 860,000 at R = 4) takes the one-sided pass.  The
-// choice depends on the simulation alone, so its sums are the same in any
-// batch.
+// j-side slabs of the mutual pass grow as 3 N^2 / (128 R W) floats per
+// simulation (W rows per slab group: 1, or 4 in the row-group form); a
+// simulation whose slab would exceed this many floats (16 GiB: about
+// N > 1.2 M at R = 8 and W = 1, 2.4 M at R = 8 and W = 4) takes the one-sided
+// pass.  The choice depends on the simulation and the tiling alone, so its
+// sums are the same in any batch.
 constexpr long long kJsideMaxFloatsPerSim = 1LL << 32;
 constexpr long long kMutualMaxChunk = 4096;
 
@@ -262,26 +268,95 @@ constexpr long long kMutualMaxChunk = 4096;
 // the row's j-range beyond it cut (at the iron/silicate border) into chunks
 // of at most L in whole 64-blocks.  Simulations left with gn = 0 (inactive,
 // empty or too large) are for the one-sided pass.
+// The row-group form of the mutual pass (variant 4): per simulation, groups
+// of W rows; each group's j-range beyond its last row is cut (at the
+// iron/silicate border) into pieces of at most L as whole 64-blocks, one
+// workgroup task each; every row has a diagonal task and, unless it is the
+// last of its group, tasks against the rest of its group (one per material
+// segment).  A row's i-side slots are its group's pieces in order, then its
+// diagonal, then its in-group tasks: contiguous, as the sums expect.
+static int plan_mutual_groups(const std::vector<SimDesc>& sims, const Tiling& tl, long long L, int W, Plan& p,
+                              std::vector<MutualTask>& tasks, std::vector<RowDesc>& rows, int& slot) {
+    const int B = 64 * tl.R;
+    for (size_t si = 0; si < sims.size(); ++si) {
+        const SimDesc& s = sims[si];
+        if (s.gn == 0) continue;
+        const int nrows = s.gn / B;
+        const int fe_real = (s.fe_end - s.start + 63) / 64 * 64;
+        const int si_real = s.gfe + (s.end - s.fe_end + 63) / 64 * 64;
+        for (int g = 0; g * W < nrows; ++g) {
+            const int r0 = g * W, r1 = std::min(r0 + W, nrows), gend = r1 * B;
+            struct Piece {
+                int a, b, jfe;
+            };
+            std::vector<Piece> pieces;
+            const int seg0[2] = {gend, std::max(gend, s.gfe)};
+            const int seg1[2] = {fe_real, si_real};
+            for (int sg = 0; sg < 2; ++sg) {
+                const int a0 = seg0[sg], a1 = seg1[sg];
+                if (a1 <= a0) continue;
+                const long long blocks = (a1 - a0) / 64;
+                long long np = (a1 - a0 + L - 1) / L;
+                if (np > blocks) np = blocks;
+                for (long long k = 0; k < np; ++k)
+                    pieces.push_back(Piece{a0 + (int)(64 * (blocks * k / np)), a0 + (int)(64 * (blocks * (k + 1) / np)),
+                                           sg == 0 ? 1 : 0});
+            }
+            for (int r = r0; r < r1; ++r) {
+                RowDesc rd{slot, 0};
+                slot += (int)pieces.size();
+                const int rb = r * B;
+                const bool rowfe = rb < s.gfe;
+                tasks.push_back(MutualTask{(int)si, r, rb, std::min(rb + B, rowfe ? fe_real : si_real), slot++,
+                                           rowfe ? 1 : 0, 1, 0});
+                // the rest of the group: iron part, then silicate part
+                const int a = rb + B, b = gend;
+                const int ia = a, ib = std::min(b, fe_real);
+                if (ia < ib) tasks.push_back(MutualTask{(int)si, r, ia, ib, slot++, 1, 0, 0});
+                const int sa = std::max(a, s.gfe), sb = std::min(b, si_real);
+                if (sa < sb) tasks.push_back(MutualTask{(int)si, r, sa, sb, slot++, 0, 0, 0});
+                rd.nslot = slot - rd.slot0;
+                rows.push_back(rd);
+            }
+            for (size_t k = 0; k < pieces.size(); ++k)
+                p.gtasks.push_back(GroupTask{0, (int)si, g, pieces[k].a, pieces[k].b, (int)k, pieces[k].jfe, 0});
+        }
+    }
+    return 0;
+}
+
+// Entries of a simulation's j-side slab (per component) with groups of W
+// rows: group g holds one entry per j beyond its last row.
+static long long group_slab_entries(long long gn, long long nrows, int B, int W) {
+    long long e = 0;
+    for (long long g = 0; g * W < nrows; ++g) e += gn - std::min((g + 1) * W, nrows) * B;
+    return e;
+}
+
 static int plan_mutual(std::vector<SimDesc>& sims, const Tiling& tl, bool batched, Plan& p) {
     const int B = 64 * tl.R;
-    long long gb = 0, jb = 0, off_total = 0, row = 0;
+    const int W = tl.variant == 4 ? kGroupRows : 1;
+    long long gb = 0, jb = 0, ji = 0, off_total = 0, row = 0;
     for (auto& s : sims) {
         s.gbase = (int)gb;
         s.row0 = (int)row;
         s.jbase = jb;
+        s.jibase = ji;
         s.gfe = s.gn = 0;
         if (s.end == s.start || !s.active) continue;
         const long long nfe = s.fe_end - s.start, nsi = s.end - s.fe_end;
         const long long gfe = (nfe + B - 1) / B * B;
         const long long gn = gfe + (nsi + B - 1) / B * B;
         const long long nrows = gn / B;
-        if (3 * (gn * (nrows - 1) / 2) > kJsideMaxFloatsPerSim) continue;
+        const long long slab = 3 * group_slab_entries(gn, nrows, B, W);
+        if (slab > kJsideMaxFloatsPerSim) continue;
         s.gfe = (int)gfe;
         s.gn = (int)gn;
         gb += gn;
         row += nrows;
         off_total += gn * (nrows - 1) / 2;
-        jb += 3 * (gn * (nrows - 1) / 2);
+        jb += slab;
+        if (W > 1) ji += 3LL * (W - 1) * B * nrows;
     }
     if (gb == 0) return 0;
     if (jb > (1LL << 35)) return set_error(GI_E_TOO_LARGE, "mutual pass: j-side slabs too large (%lld floats)", jb);
@@ -296,7 +371,11 @@ static int plan_mutual(std::vector<SimDesc>& sims, const Tiling& tl, bool batche
     std::vector<MutualTask> tasks;
     std::vector<RowDesc> rows;
     int slot = 0;
-    for (size_t si = 0; si < sims.size(); ++si) {
+    if (W > 1) {
+        int rc = plan_mutual_groups(sims, tl, L, W, p, tasks, rows, slot);
+        if (rc) return rc;
+    }
+    for (size_t si = 0; si < sims.size() && W == 1; ++si) {
         const SimDesc& s = sims[si];
         if (s.gn == 0) continue;
         const int nrows = s.gn / B;
@@ -342,7 +421,7 @@ static int plan_mutual(std::vector<SimDesc>& sims, const Tiling& tl, bool batche
             if (q0 < s.gfe ? q0 >= nfe : q0 - s.gfe >= nsi) continue;  // ghosts only
             const int I = q0 / B;
             sblocks.push_back(SumBlock{(int)si, q0});
-            swork.push_back(rows[s.row0 + I].nslot + I);
+            swork.push_back(rows[s.row0 + I].nslot + I / W + I % W);
         }
     }
     std::vector<int> order(sblocks.size());
@@ -356,13 +435,27 @@ static int plan_mutual(std::vector<SimDesc>& sims, const Tiling& tl, bool batche
     };
     std::stable_sort(tasks.begin(), tasks.end(),
                      [&](const MutualTask& a, const MutualTask& b) { return cost(a) > cost(b); });
+    if (W > 1) {
+        // the row tasks, W to a workgroup in cost order (a filler pads the
+        // last), then every workgroup longest first
+        while (tasks.size() % W) tasks.push_back(MutualTask{0, 0, 0, 0, -1, 0, 0, 0});
+        for (size_t k = 0; k < tasks.size(); k += W)
+            p.gtasks.push_back(GroupTask{1, tasks[k].sim, 0, 0, 0, 0, 0, (int)k});
+        auto gcost = [&](const GroupTask& g) {
+            return g.kind == 1 ? cost(tasks[g.rtask0]) : (long long)(g.j1 - g.j0) * (15 * tl.R + 6);
+        };
+        std::stable_sort(p.gtasks.begin(), p.gtasks.end(),
+                         [&](const GroupTask& a, const GroupTask& b) { return gcost(a) > gcost(b); });
+    }
     p.mtasks = std::move(tasks);
     p.rows = std::move(rows);
+    p.group_rows = W;
     p.gtot = gb;
     p.iside_len = (long long)slot * 3 * B;
     p.jside_len = jb;
+    p.jintra_len = ji;
     p.chunk_len = L;
-    p.mutual = !p.mtasks.empty();
+    p.mutual = W > 1 ? !p.gtasks.empty() : !p.mtasks.empty();
     return 0;
 }
 
@@ -381,7 +474,7 @@ static int build_plan(std::vector<SimDesc> sims, const Tiling& tl, bool batched,
     // One-sided tasks use the tiling's R, except beside the mutual pass, where
     // the one-sided pass is a fallback for simulations too large for it and
     // takes R = 1 (the one-sided optimum on gfx950).
-    p.plain_R = tl.variant == 2 ? 1 : tl.R;
+    p.plain_R = (tl.variant == 2 || tl.variant == 4) ? 1 : tl.R;
     const int ib = 64 * p.plain_R;
     for (auto& s : sims) {
         long long n = s.end - s.start;
@@ -391,7 +484,7 @@ static int build_plan(std::vector<SimDesc> sims, const Tiling& tl, bool batched,
     }
     // The contact-grid scan handles up to 4096 * 1024 buckets = 2 * 2^21.
     if (ntot > (1LL << 21)) return set_error(GI_E_TOO_LARGE, "too many particles (%lld > 2^21)", ntot);
-    if (tl.variant == 2) {
+    if (tl.variant == 2 || tl.variant == 4) {
         int rc = plan_mutual(sims, tl, batched, p);
         if (rc < 0) return rc;
     }
@@ -412,7 +505,7 @@ static int build_plan(std::vector<SimDesc> sims, const Tiling& tl, bool batched,
         s.partial_base = (int)pbase;
         s.nchunk = 0;
         if (n == 0 || !s.active || s.gn > 0 || p.direct) continue;  // gn > 0: mutual pass
-        const long long L = batched ? (tl.variant == 2 ? 16384 : tl.chunk_len) : sim_chunk_len(n);
+        const long long L = batched ? ((tl.variant == 2 || tl.variant == 4) ? 16384 : tl.chunk_len) : sim_chunk_len(n);
         // j segments: iron [start, fe_end), silicate [fe_end, end); each split
         // into near-equal chunks of at most L.
         const int seg0[2] = {s.start, s.fe_end};
@@ -476,6 +569,120 @@ static int build_plan(std::vector<SimDesc> sims, const Tiling& tl, bool batched,
     return 0;
 }
 
+// Every index the mutual pass's kernels (k_mutual or k_mutual_grp, then
+// k_mutual_sum) derive from a plan, checked against the buffers the plan
+// sizes, on the host: a plan that would read or write out of bounds is an
+// error before anything is launched (gi_plan_check runs it for the tests).
+static int check_plan_indexes(const Plan& p) {
+    if (!p.mutual) return 0;
+    const int R = p.tiling.R, B = 64 * R, W = p.group_rows;
+    const long long slots = p.iside_len / (3LL * B);
+#define GI_CHECK(c, ...)                                            \
+    do {                                                            \
+        if (!(c)) return set_error(GI_E_STATE, "plan check: " __VA_ARGS__); \
+    } while (0)
+    long long jside_end = 0, jintra_end = 0;
+    for (size_t si = 0; si < p.sims.size(); ++si) {
+        const SimDesc& sd = p.sims[si];
+        if (sd.gn == 0) continue;
+        const long long nrows = sd.gn / B;
+        GI_CHECK(sd.gn % B == 0 && sd.gfe % B == 0 && sd.gfe <= sd.gn, "sim %zu padded sizes", si);
+        GI_CHECK(sd.jbase >= jside_end && sd.jibase >= jintra_end, "sim %zu slab regions overlap", si);
+        long long ent = 0;
+        for (long long g = 0; g * W < nrows; ++g) ent += sd.gn - std::min((g + 1) * W, nrows) * B;
+        jside_end = sd.jbase + 3 * ent;
+        jintra_end = sd.jibase + (W > 1 ? 3LL * (W - 1) * B * nrows : 0);
+        GI_CHECK(jside_end <= p.jside_len && jintra_end <= std::max(p.jintra_len, 0LL), "sim %zu slabs exceed buffers", si);
+        GI_CHECK(sd.gbase + sd.gn <= p.gtot && sd.row0 + nrows <= (long long)p.rows.size(), "sim %zu rows", si);
+        for (long long rr = 0; rr < nrows; ++rr) {
+            const RowDesc& rd = p.rows[sd.row0 + rr];
+            GI_CHECK(rd.slot0 >= 0 && rd.nslot >= 1 && rd.slot0 + rd.nslot <= slots, "sim %zu row %lld slots", si, rr);
+        }
+    }
+    // one wavefront's row task: k_mutual's J (W = 1) or the in-group slab (W > 1)
+    auto check_row_task = [&](const MutualTask& t) -> int {
+        GI_CHECK(t.sim >= 0 && t.sim < (int)p.sims.size(), "task sim");
+        const SimDesc& sd = p.sims[t.sim];
+        const long long nrows = sd.gn / B;
+        GI_CHECK(t.row >= 0 && t.row < nrows, "task row %d of %lld", t.row, nrows);
+        GI_CHECK(t.slot >= 0 && t.slot < slots, "task slot %d", t.slot);
+        const RowDesc& rd = p.rows[sd.row0 + t.row];
+        GI_CHECK(t.slot >= rd.slot0 && t.slot < rd.slot0 + rd.nslot, "task slot outside its row's");
+        GI_CHECK(t.j0 % 64 == 0 && t.j1 % 64 == 0 && 0 <= t.j0 && t.j0 < t.j1 && t.j1 <= sd.gn, "task j-range");
+        const long long ib = (long long)t.row * B;
+        if (t.diag) {
+            GI_CHECK(t.j0 == ib && t.j1 <= ib + B, "diagonal task j-range");
+            return 0;
+        }
+        const long long first = ib + B;
+        GI_CHECK(t.j0 >= first, "row task reaches back into its row");
+        GI_CHECK((t.jfe != 0) == (t.j1 <= sd.gfe) && (t.jfe != 0 || t.j0 >= sd.gfe), "task material");
+        if (W == 1) {
+            GI_CHECK(sd.jbase + 3 * jside_row_offset(t.row, sd.gn, B) + 3 * (sd.gn - first) <= p.jside_len, "J");
+        } else {
+            const long long ilen = (long long)(W - 1) * B;
+            GI_CHECK(t.j1 - first <= ilen, "in-group task beyond its group");
+            GI_CHECK(t.j1 <= std::min(((long long)t.row / W + 1) * W, nrows) * B, "in-group task beyond its group");
+            GI_CHECK(sd.jibase + 3 * ilen * (t.row + 1) <= p.jintra_len, "in-group slab");
+        }
+        return 0;
+    };
+    int rc;
+    if (W == 1) {
+        for (const MutualTask& t : p.mtasks)
+            if ((rc = check_row_task(t))) return rc;
+    } else {
+        for (const GroupTask& g : p.gtasks) {
+            if (g.kind == 1) {
+                GI_CHECK(g.rtask0 >= 0 && g.rtask0 % W == 0 && g.rtask0 + W <= (int)p.mtasks.size(), "row bundle");
+                for (int w = 0; w < W; ++w) {
+                    const MutualTask& t = p.mtasks[g.rtask0 + w];
+                    if (t.j1 > t.j0 && (rc = check_row_task(t))) return rc;
+                }
+                continue;
+            }
+            GI_CHECK(g.kind == 0 && g.sim >= 0 && g.sim < (int)p.sims.size(), "group task kind/sim");
+            const SimDesc& sd = p.sims[g.sim];
+            const long long nrows = sd.gn / B;
+            GI_CHECK(g.group >= 0 && (long long)g.group * W < nrows, "group %d of %lld rows", g.group, nrows);
+            const long long gend = std::min(((long long)g.group + 1) * W, nrows) * B;
+            GI_CHECK(g.j0 % 64 == 0 && g.j1 % 64 == 0 && gend <= g.j0 && g.j0 < g.j1 && g.j1 <= sd.gn,
+                     "group task j-range [%d, %d) after %lld", g.j0, g.j1, gend);
+            GI_CHECK((g.jfe != 0) == (g.j1 <= sd.gfe) && (g.jfe != 0 || g.j0 >= sd.gfe), "group task material");
+            const long long len = sd.gn - gend;
+            GI_CHECK(sd.jbase + 3 * jside_row_offset(g.group, sd.gn, W * B) + 3 * len <= p.jside_len, "group J");
+            for (long long rr = (long long)g.group * W; rr < std::min(((long long)g.group + 1) * W, nrows); ++rr) {
+                const RowDesc& rd = p.rows[sd.row0 + rr];
+                GI_CHECK(g.piece >= 0 && g.piece < rd.nslot, "group piece %d, row %lld has %d slots", g.piece, rr,
+                         rd.nslot);
+            }
+        }
+    }
+    // k_mutual_sum: every real particle's slots and slab entries
+    for (const SumBlock& b : p.sblocks) {
+        GI_CHECK(b.sim >= 0 && b.sim < (int)p.sims.size(), "sum block sim");
+        const SimDesc& sd = p.sims[b.sim];
+        GI_CHECK(b.q0 % 64 == 0 && b.q0 + 64 <= sd.gn, "sum block places");
+        const int row = b.q0 / B, group = row / W;
+        const long long WB = (long long)W * B, ilen = (long long)(W - 1) * B;
+        for (int q = b.q0; q < b.q0 + 64; ++q) {
+            for (int g = 0; g < group; ++g) {
+                const long long len = sd.gn - (g + 1) * WB;
+                const long long o = q - (g + 1) * WB;
+                GI_CHECK(o >= 0 && o < len, "sum: group entry");
+                GI_CHECK(sd.jbase + 3 * jside_row_offset(g, sd.gn, WB) + 2 * len + o < p.jside_len, "sum: J");
+            }
+            for (int r2 = group * W; r2 < row; ++r2) {
+                const long long o = q - (long long)(r2 + 1) * B;
+                GI_CHECK(o >= 0 && o < ilen && sd.jibase + 3 * ilen * r2 + 2 * ilen + o < p.jintra_len,
+                         "sum: in-group entry");
+            }
+        }
+    }
+#undef GI_CHECK
+    return 0;
+}
+
 template <typename T>
 static int upload(DevBuf<T>& b, const std::vector<T>& v, hipStream_t st) {
     int rc = b.ensure(v.size());
@@ -507,10 +714,12 @@ static int ensure_plan(Workspace* ws, const std::vector<SimDesc>& sims, hipStrea
         if ((rc = ws->gsum.ensure((size_t)p.ntot + 1))) return rc;
         if ((rc = ws->iside.ensure((size_t)p.iside_len + 1))) return rc;
         if ((rc = ws->jside.ensure((size_t)p.jside_len + 1))) return rc;
+        if ((rc = ws->jintra.ensure((size_t)p.jintra_len + 1))) return rc;
+        if ((rc = upload(ws->gtasks, p.gtasks, st))) return rc;
         if ((rc = upload(ws->mtasks, p.mtasks, st))) return rc;
         if ((rc = upload(ws->rows, p.rows, st))) return rc;
         if ((rc = upload(ws->sblocks, p.sblocks, st))) return rc;
-        if (p.split_finish && (rc = ws->corr.ensure((size_t)3 * p.ntot + 1))) return rc;
+        if (p.split_finish && (rc = ws->corr.ensure((size_t)6 * p.ntot + 1))) return rc;
     }
     if (p.direct && (rc = upload(ws->dblocks, p.dblocks, st))) return rc;
     if ((rc = ws->counts.ensure((size_t)p.buckets + 1))) return rc;
@@ -603,6 +812,19 @@ static void launch_gravity_kernel(int R, hipStream_t st, const float4* pos, cons
                            ws->partial.p);
 }
 
+// Workgroup tasks [t0, t0 + ntasks) of the plan's row-group list.
+template <bool POT>
+static void launch_mutual_group_kernel(int R, hipStream_t st, const Workspace* ws, int t0, int ntasks) {
+    const dim3 grid(ntasks), block(64 * kMutualWaves);
+    const GroupTask* tasks = ws->gtasks.p + t0;
+    if (R == 8)
+        hipLaunchKernelGGL((k_mutual_grp<8, POT>), grid, block, 0, st, ws->gpos.p, tasks, ntasks, ws->mtasks.p,
+                           ws->sims.p, ws->rows.p, ws->iside.p, ws->jside.p, ws->jintra.p);
+    else
+        hipLaunchKernelGGL((k_mutual_grp<4, POT>), grid, block, 0, st, ws->gpos.p, tasks, ntasks, ws->mtasks.p,
+                           ws->sims.p, ws->rows.p, ws->iside.p, ws->jside.p, ws->jintra.p);
+}
+
 // Tasks [t0, t0 + ntasks) of the plan's mutual list.
 template <bool POT>
 static void launch_mutual_kernel(int R, hipStream_t st, const Workspace* ws, int t0, int ntasks) {
@@ -639,7 +861,8 @@ static int launch_gravity(DevCtx* ctx, Workspace* ws, const float4* pos, bool po
                           float4* mvel = nullptr, const SumMove& mv = SumMove{0.0f, 0.0f, 0.0f, 1}) {
     const Plan& p = ws->plan;
     const int nplain = (int)p.tasks.size();
-    const int nmut = (int)p.mtasks.size();
+    const bool grouped = p.group_rows > 1;
+    const int nmut = grouped ? (int)p.gtasks.size() : (int)p.mtasks.size();
     int rc;
     if (nplain == 0 && nmut == 0) {
         if (fork) GI_HIP(hipEventRecord(fork, st));
@@ -677,7 +900,11 @@ static int launch_gravity(DevCtx* ctx, Workspace* ws, const float4* pos, bool po
     const int step_mut = lim > 0 && lim < nmut ? (int)lim : std::max(nmut, 1);
     for (int t0 = 0; t0 < nmut; t0 += step_mut) {
         const int cnt = std::min(step_mut, nmut - t0);
-        if (pot)
+        if (grouped && pot)
+            launch_mutual_group_kernel<true>(p.tiling.R, st, ws, t0, cnt);
+        else if (grouped)
+            launch_mutual_group_kernel<false>(p.tiling.R, st, ws, t0, cnt);
+        else if (pot)
             launch_mutual_kernel<true>(p.tiling.R, st, ws, t0, cnt);
         else
             launch_mutual_kernel<false>(p.tiling.R, st, ws, t0, cnt);
@@ -690,10 +917,16 @@ static int launch_gravity(DevCtx* ctx, Workspace* ws, const float4* pos, bool po
     if (side_work && (rc = side_work())) return rc;
     if (join) GI_HIP(hipStreamWaitEvent(st, join, 0));
     if (nmut > 0 && !p.sblocks.empty()) {
-        hipLaunchKernelGGL(k_mutual_sum, dim3((unsigned)p.sblocks.size()), dim3(64 * kSumSplit), 0, st,
-                           ws->sblocks.p, ws->sims.p, ws->rows.p, 64 * p.tiling.R, ws->iside.p, ws->jside.p,
-                           ws->gsum.p, pot ? 1 : 0, force ? (const double*)ws->corr.p : nullptr, p.ntot, force, mpos,
-                           mvel, mv);
+        const dim3 grid((unsigned)p.sblocks.size()), block(64 * kSumSplit);
+        const float* corr = force ? (const float*)ws->corr.p : nullptr;
+        if (pot)
+            hipLaunchKernelGGL(k_mutual_sum<true>, grid, block, 0, st, ws->sblocks.p, ws->sims.p, ws->rows.p,
+                               64 * p.tiling.R, p.group_rows, ws->iside.p, ws->jside.p, ws->jintra.p, ws->gsum.p,
+                               corr, p.ntot, force, mpos, mvel, mv);
+        else
+            hipLaunchKernelGGL(k_mutual_sum<false>, grid, block, 0, st, ws->sblocks.p, ws->sims.p, ws->rows.p,
+                               64 * p.tiling.R, p.group_rows, ws->iside.p, ws->jside.p, ws->jintra.p, ws->gsum.p,
+                               corr, p.ntot, force, mpos, mvel, mv);
         GI_HIP(hipGetLastError());
     }
     if (timed) {
@@ -1235,18 +1468,35 @@ int gi_tiling_for(long long n, int* r, int* variant, long long* chunk_len, long 
     return 0;
 }
 
+int gi_plan_check(int n_sims, const int* n, const int* nfe, int r, int variant, long long chunk_len, int batched) {
+    if (n_sims <= 0 || !n || !nfe || (r != 1 && r != 2 && r != 4 && r != 8) || (variant != 2 && variant != 4) ||
+        (variant == 4 && r < 4) || chunk_len < 0)
+        return set_error(GI_E_INVALID_ARG, "gi_plan_check: invalid arguments");
+    std::vector<SimDesc> sims;
+    int off = 0;
+    for (int s = 0; s < n_sims; ++s) {
+        if (n[s] < 0 || nfe[s] < 0 || nfe[s] > n[s]) return set_error(GI_E_INVALID_ARG, "gi_plan_check: sim %d", s);
+        sims.push_back(make_sim(off, off + nfe[s], off + n[s], PairLaw{}));
+        off += n[s];
+    }
+    Plan p;
+    int rc = build_plan(sims, Tiling{r, variant, chunk_len, 0, 64}, batched != 0, &p);
+    if (rc) return rc;
+    return check_plan_indexes(p);
+}
+
 int gi_plan_info(int* mutual, int* r, long long* pair_tasks, long long* chunk_len) {
     DevCtx* ctx;
     int rc = current_ctx(&ctx);
     if (rc) return rc;
     const Plan& p = ctx->ws.plan;
     const bool valid = ctx->ws.plan_valid;
-    if (mutual) *mutual = !valid ? 0 : (p.direct ? 2 : (p.mutual ? 1 : 0));
+    if (mutual) *mutual = !valid ? 0 : (p.direct ? 2 : (p.mutual ? (p.group_rows > 1 ? 3 : 1) : 0));
     if (r) *r = valid ? p.tiling.R : 0;
     if (pair_tasks)
         *pair_tasks = !valid     ? 0
                       : p.direct ? (long long)p.dblocks.size()
-                                 : (long long)(p.tasks.size() + p.mtasks.size());
+                                 : (long long)(p.tasks.size() + (p.group_rows > 1 ? p.gtasks.size() : p.mtasks.size()));
     if (chunk_len) *chunk_len = valid ? p.chunk_len : 0;
     return 0;
 }
@@ -1262,9 +1512,10 @@ int gi_set_batched_tiling(int r, long long chunk_len, int variant) {
         batched_override() = BatchedTiling{0, 0, 0};
         return 0;
     }
-    if ((r != 1 && r != 2 && r != 4 && r != 8) || chunk_len < 64 || variant < 0 || variant > 2)
+    if ((r != 1 && r != 2 && r != 4 && r != 8) || chunk_len < 64 || variant < 0 || variant == 3 || variant > 4 ||
+        (variant == 4 && r < 4))
         return set_error(GI_E_INVALID_ARG,
-                         "gi_set_batched_tiling: r in {1,2,4,8}, chunk_len >= 64, variant in {0,1,2}");
+                         "gi_set_batched_tiling: r in {1,2,4,8}, chunk_len >= 64, variant in {0,1,2,4} (4: r >= 4)");
     batched_override() = BatchedTiling{(chunk_len + 63) / 64 * 64, r, variant};
     return 0;
 }
@@ -1287,8 +1538,8 @@ int gi_set_tiling(int r, int variant, long long chunk_len, long long target_task
     }
     if (r != 1 && r != 2 && r != 4 && r != 8)
         return set_error(GI_E_INVALID_ARG, "gi_set_tiling: r must be 1, 2, 4 or 8");
-    if (variant < 0 || variant > 3)
-        return set_error(GI_E_INVALID_ARG, "gi_set_tiling: variant must be 0, 1, 2 or 3");
+    if (variant < 0 || variant > 4 || (variant == 4 && r < 4))
+        return set_error(GI_E_INVALID_ARG, "gi_set_tiling: variant must be 0, 1, 2, 3 or 4 (4 needs r >= 4)");
     if (chunk_len < 0 || target_tasks < 0 || min_chunk < 0)
         return set_error(GI_E_INVALID_ARG, "gi_set_tiling: negative size");
     tiling_override().active = 1;

@@ -31,9 +31,11 @@ struct SimDesc {
     float pad2;
     // Mutual all-pairs pass: the simulation's copy in the padded layout
     // (iron then silicate, each padded with ghosts to a whole number of
-    // 64*R rows), its first row in the row table and its j-side slab.
+    // 64*R rows), its first row in the row table, its j-side slab and (the
+    // grouped form) its within-group slab.
     int gbase, gfe, gn, row0;
     long long jbase;
+    long long jibase;
 };
 
 // One wavefront of the mutual all-pairs pass: row `row` (64*R resident
@@ -42,6 +44,17 @@ struct SimDesc {
 // otherwise j lies beyond the row and each pair is evaluated once for both.
 struct MutualTask {
     int sim, row, j0, j1, slot, jfe, diag, pad;
+};
+
+// One workgroup of the grouped mutual pass.  kind 0: the rows group*W ..
+// group*W + W - 1 of simulation sim (wave w holds row group*W + w; waves past
+// the simulation's last row only keep step) against j in [j0, j1) beyond the
+// group, all of material jfe; `piece` numbers this j-range among the group's,
+// so each row's i-side slot is rows[row].slot0 + piece.  kind 1: W
+// independent single-wave row tasks rtask0 .. rtask0 + W - 1 of the row-task
+// table (a row against itself, or against the rest of its group).
+struct GroupTask {
+    int kind, sim, group, j0, j1, piece, jfe, rtask0;
 };
 
 // One workgroup of the direct pass: 64 i-particles [i0, i0 + 64) of
@@ -142,20 +155,58 @@ __device__ __forceinline__ float approach_k(const PairLaw& L, int test) {
     return test == 0 ? L.KSiSi : (test == 1 ? L.KFeSi : L.KFeFe);
 }
 
+// Compensated float sum (Neumaier's variant of Kahan summation): the
+// rounding error of every addition, whichever operand is the larger, is
+// carried in c, so a long sum of floats keeps float precision of its result
+// instead of losing it with the number of terms.  The force path accumulates
+// in it; FP64 is kept for the energy and angular-momentum reductions.
+struct CompSum {
+    float s, c;
+    __device__ __forceinline__ void zero() { s = c = 0.0f; }
+    __device__ __forceinline__ void add(float x) {
+        const float t = s + x;
+        c += fabsf(s) >= fabsf(x) ? (s - t) + x : (x - t) + s;
+        s = t;
+    }
+    // another compensated sum, its compensation carried over
+    __device__ __forceinline__ void add_sum(const CompSum& o) {
+        add(o.s);
+        c += o.c;
+    }
+    // the product a * b, its rounding error (exact, by FMA) carried over
+    __device__ __forceinline__ void add_product(float a, float b) {
+        const float p = a * b;
+        add(p);
+        c += __builtin_fmaf(a, b, -p);
+    }
+    __device__ __forceinline__ float value() const { return s + c; }
+};
+
 // r^-3 as the all-pairs pass computes it (one hardware reciprocal square
 // root, two multiplies) from r^2 + eps accumulated with FMAs.
+//
+// The reciprocal root is clamped to [0, 1] (fmed3 against 0 and 1 folds into
+// the output clamp bit of v_rsq_f32 / v_rcp_f32: no extra instruction).  For
+// r >= 1 that leaves the value as it is; for r < 1 -- a pair in contact,
+// whose term the contact correction takes out again and replaces by the
+// contact law -- the term becomes the bounded placeholder g dp (|dp| < 1)
+// instead of g dp / r^3.  The float sums of the all-pairs pass therefore
+// never carry a 1/r^2-sized term of a close pair, whose rounding would
+// survive its exact removal; the self pair (r^2 = kSelfEps) gives 1 * 0 = 0.
+// Every correction subtracts this same function's value for the pair.
 __device__ __forceinline__ float fast_r2(float dx, float dy, float dz) {
     return __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, __builtin_fmaf(dx, dx, kSelfEps)));
 }
+__device__ __forceinline__ float clamp_unit(float x) { return __builtin_amdgcn_fmed3f(x, 0.0f, 1.0f); }
 __device__ __forceinline__ float fast_inv_r3(float r2) {
-    float ri = __builtin_amdgcn_rsqf(r2);
+    const float ri = clamp_unit(__builtin_amdgcn_rsqf(r2));
     return ri * ri * ri;
 }
 
 // Variant 1: r^-3 = rsq(r^2) * rcp(r^2): one multiply and two transcendental
-// ops, which issue beside the ordinary VALU stream.
+// ops, which issue beside the ordinary VALU stream (both clamped as above).
 __device__ __forceinline__ float fast_inv_r3_rcp(float r2) {
-    return __builtin_amdgcn_rsqf(r2) * __builtin_amdgcn_rcpf(r2);
+    return clamp_unit(__builtin_amdgcn_rsqf(r2)) * clamp_unit(__builtin_amdgcn_rcpf(r2));
 }
 
 template <int V>

@@ -11,7 +11,7 @@
 //      j-particles through wave-uniform scalar loads, Kahan-folded float
 //      tiles, one j-chunk per wavefront, partial sums to a slot buffer), or
 //      the mutual pass (k_mutual_pad, k_mutual, k_mutual_sum: each pair
-//      evaluated once for both particles, per-particle sums in FP64).
+//      evaluated once for both particles, per-particle compensated sums).
 //   2. the contact grid -- a spatial hash of cells of size 1 (the element
 //      diameter) built by counting sort (k_grid_*).
 //   3. k_finish   -- per particle: its all-pairs sum plus, for every pair
@@ -25,6 +25,8 @@
 // so the sum equals the reference's force up to float rounding, and the
 // contact branch decisions are made with the reference's own arithmetic.
 #include "gi_device.h"
+
+#include <type_traits>
 
 namespace gi {
 
@@ -162,8 +164,9 @@ __device__ __forceinline__ int find_sim(const SimDesc* __restrict__ sims, int ns
 // 64-block is of one material and the gravity product is one factor per task.
 // The row against itself ("diag" task) runs one-sided with wave-uniform
 // scalar loads.  Outputs: per task an i-side slot (3 planes of 64*R floats)
-// and per (row, j-block) a j-side slab entry; k_mutual_sum adds them in FP64
-// in a fixed order.  Both sides accumulate in plain float: a j-side entry sums
+// and per (row, j-block) a j-side slab entry; k_mutual_sum adds them in
+// compensated float sums in a fixed order.  Both sides accumulate in plain
+// float: a j-side entry sums
 // 64*R terms, an i-side slot at most one chunk (<= 4096 j, the planner's
 // cap), so the rounding error stays at a few 1e-6 of the sum of the terms'
 // magnitudes; leaving out tile sums and compensation keeps R = 8 at 94 VGPRs
@@ -184,26 +187,22 @@ __device__ __forceinline__ void wave_lds_sync() {
 
 // Offset (in entries, per plane) of row `row`'s part of a simulation's
 // j-side slab: rows before it hold gn - (r+1)*B entries each.
-__device__ __forceinline__ long long jside_row_offset(int row, int gn, int B) {
+__host__ __device__ __forceinline__ long long jside_row_offset(int row, int gn, int B) {
     return (long long)row * gn - (long long)B * row * (row + 1) / 2;
 }
 
 constexpr int kMutualWaves = 4;  // wavefronts per block of k_mutual
 
+// One wavefront's task t of simulation sd (P: its padded positions, q: the
+// wavefront's 128 float4 of LDS): a row against itself (t.diag) or against
+// j beyond the row, whose j-side sums go to J (three planes of len entries,
+// entry j - (t.row + 1) * B); the row's i-side sums go to slot t.slot.
 template <int R, bool POT>
-__global__ __launch_bounds__(64 * kMutualWaves) void k_mutual(const float4* __restrict__ gpos,
-                                                              const MutualTask* __restrict__ tasks,
-                                                              int ntasks, const SimDesc* __restrict__ sims,
-                                                              float* __restrict__ iside,
-                                                              float* __restrict__ jside) {
+__device__ __forceinline__ void mutual_wave_task(const MutualTask& t, const SimDesc& sd,
+                                                 const float4* __restrict__ P, float4* __restrict__ q,
+                                                 float* __restrict__ iside, float* __restrict__ J, long long len,
+                                                 int lane) {
     constexpr int B = 64 * R;
-    const int lane = threadIdx.x & 63;
-    const int wv = threadIdx.x >> 6;
-    const int w = __builtin_amdgcn_readfirstlane(blockIdx.x * kMutualWaves + wv);
-    if (w >= ntasks) return;
-    const MutualTask t = tasks[w];
-    const SimDesc& sd = sims[t.sim];
-    const float4* __restrict__ P = gpos + sd.gbase;
     const int ib = t.row * B;
     const bool ife = ib < sd.gfe;
     const float g = t.jfe ? (ife ? sd.law.GMassFeFe : sd.law.GMassFeSi) : (ife ? sd.law.GMassFeSi : 1.0f);
@@ -223,9 +222,6 @@ __global__ __launch_bounds__(64 * kMutualWaves) void k_mutual(const float4* __re
     // j_((127 - m) mod 64), so step k reads q[63 - lane + k] =
     // j_((lane - k) mod 64) -- the j whose accumulator lane `lane` holds
     // after k rotations -- at the immediate offset 16 k.
-    __shared__ float4 lq[kMutualWaves][128];
-    float4* __restrict__ q = lq[wv];
-
     if (R > 4 && t.diag) {
         // The row against itself, one-sided (includes the self pair): at
         // R = 8 this keeps the off-diagonal loop's register allocation (the
@@ -315,8 +311,6 @@ __global__ __launch_bounds__(64 * kMutualWaves) void k_mutual(const float4* __re
         }
     } else {
         const int first = (t.row + 1) * B;  // first j of the row's slab
-        const long long len = sd.gn - first;
-        float* __restrict__ J = jside + sd.jbase + 3 * jside_row_offset(t.row, sd.gn, B);
         float4 nx = P[t.j0 + lane];
         for (int J0 = t.j0; J0 < t.j1; J0 += 64) {
             float xj = nx.x, yj = nx.y, zj = nx.z;
@@ -385,6 +379,186 @@ __global__ __launch_bounds__(64 * kMutualWaves) void k_mutual(const float4* __re
     }
 }
 
+template <int R, bool POT>
+__global__ __launch_bounds__(64 * kMutualWaves) void k_mutual(const float4* __restrict__ gpos,
+                                                              const MutualTask* __restrict__ tasks,
+                                                              int ntasks, const SimDesc* __restrict__ sims,
+                                                              float* __restrict__ iside,
+                                                              float* __restrict__ jside) {
+    constexpr int B = 64 * R;
+    const int lane = threadIdx.x & 63;
+    const int wv = threadIdx.x >> 6;
+    const int w = __builtin_amdgcn_readfirstlane(blockIdx.x * kMutualWaves + wv);
+    if (w >= ntasks) return;
+    const MutualTask t = tasks[w];
+    const SimDesc& sd = sims[t.sim];
+    __shared__ float4 lq[kMutualWaves][128];
+    float* __restrict__ J = jside + sd.jbase + 3 * jside_row_offset(t.row, sd.gn, B);
+    mutual_wave_task<R, POT>(t, sd, gpos + sd.gbase, lq[wv], iside, J, sd.gn - (long long)(t.row + 1) * B, lane);
+}
+
+// ---------------------------------------------------------------------------
+// Pass 1, mutual form with row groups: the j-side sums of kMutualWaves rows
+// added in LDS before they are written
+// ---------------------------------------------------------------------------
+//
+// In k_mutual every (row, 64-block) pair leaves a j-side slab entry, so the
+// slab holds ~N^2 / (128 R) floats per component -- more than the whole pass
+// writes otherwise.  Here a workgroup's W = kMutualWaves wavefronts hold W
+// consecutive rows (a group) and stream the same j-blocks beyond the group:
+// the block's positions sit once in LDS for all of them, and after each block
+// the W j-side sums (each times its own row's gravity product) are added in a
+// fixed order and written as one entry per (group, block), a W-th of the
+// entries.  Pairs inside a group (a row against itself and against the later
+// rows of its group) are single-wavefront row tasks as in k_mutual, packed W
+// to a workgroup, whose j-side sums go to a small per-row slab.
+constexpr int kGroupRows = kMutualWaves;
+
+template <int R, bool POT>
+__device__ __forceinline__ void mutual_group_task(const GroupTask& gt, const SimDesc& sd,
+                                                  const RowDesc* __restrict__ rows, const float4* __restrict__ P,
+                                                  float* __restrict__ iside, float* __restrict__ jside, int wv,
+                                                  int lane, float4 (*__restrict__ sq)[128],
+                                                  float (*__restrict__ red)[3][kGroupRows][64]) {
+    constexpr int B = 64 * R;
+    constexpr int W = kGroupRows;
+    const int nrows = sd.gn / B;
+    const int row = gt.group * W + wv;
+    const bool live = row < nrows;  // wave-uniform
+    const int ib = row * B;
+    const bool ife = ib < sd.gfe;
+    const float g = !live ? 0.0f
+                          : gt.jfe ? (ife ? sd.law.GMassFeFe : sd.law.GMassFeSi) : (ife ? sd.law.GMassFeSi : 1.0f);
+    float xi[R], yi[R], zi[R];
+    float ax[R], ay[R], az[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const float4 p = live ? P[ib + r * 64 + lane] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        xi[r] = p.x;
+        yi[r] = p.y;
+        zi[r] = p.z;
+        ax[r] = ay[r] = az[r] = 0.0f;
+    }
+    const int gend = min((gt.group + 1) * W, nrows) * B;  // first j of the group's slab
+    const long long len = sd.gn - gend;
+    float* __restrict__ J = jside + sd.jbase + 3 * jside_row_offset(gt.group, sd.gn, W * B);
+    // Each wavefront stages a quarter of every block: lanes 0..15 of wave wv
+    // own j = 16 wv + lane, stored twice as k_mutual lays a block out.
+    const int jl = wv * 16 + (lane & 15);
+    const bool stager = lane < 16;
+    float4 nx = stager ? P[gt.j0 + jl] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (stager) {
+        sq[0][63 - jl] = nx;
+        sq[0][127 - jl] = nx;
+    }
+    __syncthreads();
+    int buf = 0;
+    for (int J0 = gt.j0; J0 < gt.j1; J0 += 64) {
+        const bool more = J0 + 64 < gt.j1;
+        if (more && stager) nx = P[J0 + 64 + jl];  // the next block's quarter, in flight
+        float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+        if (live) {
+            const float4* __restrict__ qb = sq[buf] + (63 - lane);
+            const float4 c0 = qb[0];
+            float xj = c0.x, yj = c0.y, zj = c0.z;
+#pragma unroll 8
+            for (int k = 0; k < 64; ++k) {
+                const float4 pn = qb[k + 1 < 64 ? k + 1 : 63];  // next step's j
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const float dx = xj - xi[r];
+                    const float dy = yj - yi[r];
+                    const float dz = zj - zi[r];
+                    if constexpr (!POT) {
+                        const float s = fast_inv_r3(fast_r2(dx, dy, dz));
+                        ax[r] = __builtin_fmaf(s, dx, ax[r]);
+                        ay[r] = __builtin_fmaf(s, dy, ay[r]);
+                        az[r] = __builtin_fmaf(s, dz, az[r]);
+                        fx = __builtin_fmaf(s, dx, fx);
+                        fy = __builtin_fmaf(s, dy, fy);
+                        fz = __builtin_fmaf(s, dz, fz);
+                    } else {
+                        const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+                        const float ri = (r2 > 0.0f && r2 < kGhostR2) ? __builtin_amdgcn_rsqf(r2) : 0.0f;
+                        ax[r] += ri;
+                        fx += ri;
+                    }
+                }
+                xj = pn.x;
+                yj = pn.y;
+                zj = pn.z;
+                fx = rotate_lanes(fx);
+                if constexpr (!POT) {
+                    fy = rotate_lanes(fy);
+                    fz = rotate_lanes(fz);
+                }
+            }
+        }
+        // This wavefront's j-side sums, times its own gravity product (the
+        // force on j is the negated sum; the potential is symmetric).
+        if constexpr (!POT) {
+            red[buf][0][wv][lane] = -fx * g;
+            red[buf][1][wv][lane] = -fy * g;
+            red[buf][2][wv][lane] = -fz * g;
+        } else {
+            red[buf][0][wv][lane] = fx * g;
+        }
+        if (more && stager) {
+            sq[buf ^ 1][63 - jl] = nx;
+            sq[buf ^ 1][127 - jl] = nx;
+        }
+        __syncthreads();
+        // wavefront c adds component c over the group's rows in row order
+        if (wv < (POT ? 1 : 3)) {
+            float a = red[buf][wv][0][lane];
+#pragma unroll
+            for (int w = 1; w < W; ++w) a += red[buf][wv][w][lane];
+            J[wv * len + (J0 - gend) + lane] = a;
+        }
+        buf ^= 1;
+    }
+    if (live) {
+        float* __restrict__ I = iside + (size_t)(rows[sd.row0 + row].slot0 + gt.piece) * (3 * B);
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            I[r * 64 + lane] = ax[r] * g;
+            if constexpr (!POT) {
+                I[B + r * 64 + lane] = ay[r] * g;
+                I[2 * B + r * 64 + lane] = az[r] * g;
+            }
+        }
+    }
+}
+
+template <int R, bool POT>
+__global__ __launch_bounds__(64 * kMutualWaves) void k_mutual_grp(const float4* __restrict__ gpos,
+                                                                  const GroupTask* __restrict__ gtasks, int ntasks,
+                                                                  const MutualTask* __restrict__ rtasks,
+                                                                  const SimDesc* __restrict__ sims,
+                                                                  const RowDesc* __restrict__ rows,
+                                                                  float* __restrict__ iside,
+                                                                  float* __restrict__ jside,
+                                                                  float* __restrict__ jintra) {
+    constexpr int B = 64 * R;
+    const int lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if ((int)blockIdx.x >= ntasks) return;
+    const GroupTask gt = gtasks[blockIdx.x];
+    __shared__ float4 lq[kMutualWaves][128];
+    __shared__ float red[2][3][kGroupRows][64];
+    if (gt.kind == 1) {
+        const MutualTask t = rtasks[gt.rtask0 + wv];
+        if (t.j1 <= t.j0) return;  // a filler of the last workgroup
+        const SimDesc& sd = sims[t.sim];
+        float* __restrict__ J = jintra + sd.jibase + 3LL * (kGroupRows - 1) * B * t.row;
+        mutual_wave_task<R, POT>(t, sd, gpos + sd.gbase, lq[wv], iside, J, (long long)(kGroupRows - 1) * B, lane);
+        return;
+    }
+    const SimDesc& sd = sims[gt.sim];
+    mutual_group_task<R, POT>(gt, sd, rows, gpos + sd.gbase, iside, jside, wv, lane,
+                              reinterpret_cast<float4(*)[128]>(&lq[0][0]), red);
+}
+
 // Binary search of the simulation owning padded slot q (sims ordered by gbase).
 __device__ __forceinline__ int find_gsim(const SimDesc* __restrict__ sims, int nsims, int q) {
     int lo = 0, hi = nsims - 1;
@@ -416,15 +590,17 @@ __global__ void k_mutual_pad(const float4* __restrict__ pos, int gtot, const Sim
 }
 
 // Per particle: the i-side slots of its row and the j-side entries of every
-// earlier row, summed in FP64 in a fixed order.  A workgroup takes 64 padded
+// earlier row (k_mutual), or of every earlier group and the earlier rows of
+// its own group (k_mutual_grp, groups of W rows), added in compensated
+// float sums (CompSum) in a fixed order.  A workgroup takes 64 padded
 // places of one row (a SumBlock; the host lists the blocks holding a real
 // particle, heaviest rows first), so the loop bounds are uniform.  Four
 // wavefronts per block: wavefront w adds the w-th quarter of each particle's
 // slots and slab entries (one particle per lane, so the loads stay
 // coalesced), and the quarters are added in LDS in a fixed order -- four
 // times the loads in flight of one thread per particle.
-// With corr (the contact corrections of k_contact_corr), the sum rounded to
-// float plus the correction goes to force.xyz -- k_finish's arithmetic --
+// With corr (the contact corrections of k_contact_corr), the correction plus
+// the sum rounded to float goes to force.xyz -- k_finish's arithmetic --
 // and, with mpos / mvel, the particle then takes the move's step (k_move's
 // and k_move_collision's arithmetic) with the constants of mv, or with its
 // simulation's own when mv.per_sim; without corr, the sum goes to gsum for
@@ -437,24 +613,73 @@ struct SumMove {
     int per_sim;  // 1: take them from each SimDesc (the batched engine)
 };
 
+// Entries k0 .. k1 - 1 of one list (slots, or slab entries) added to
+// (sx, sy, sz) in order.  at(k, p, plane) gives the wavefront-uniform address
+// of entry k's x value for lane 0 (lane l reads p[l]) and the distance to its
+// y and z values.  The loads of kSumBatch entries are issued before their
+// additions, so a wavefront has that many round trips to memory in flight
+// instead of one; the additions keep their order, so the sums are those of
+// the one-at-a-time loop.  POT: x only.
+constexpr int kSumBatch = 8;
+
+template <bool POT, typename At>
+__device__ __forceinline__ void add_entries(int k0, int k1, int lane, At at, CompSum& sx, CompSum& sy,
+                                            CompSum& sz) {
+    int k = k0;
+    for (; k + kSumBatch <= k1; k += kSumBatch) {
+        float vx[kSumBatch], vy[kSumBatch], vz[kSumBatch];
+#pragma unroll
+        for (int u = 0; u < kSumBatch; ++u) {
+            const float* p;
+            long long plane;
+            at(k + u, p, plane);
+            vx[u] = p[lane];
+            if constexpr (!POT) {
+                vy[u] = p[plane + lane];
+                vz[u] = p[2 * plane + lane];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kSumBatch; ++u) {
+            sx.add(vx[u]);
+            if constexpr (!POT) {
+                sy.add(vy[u]);
+                sz.add(vz[u]);
+            }
+        }
+    }
+    for (; k < k1; ++k) {
+        const float* p;
+        long long plane;
+        at(k, p, plane);
+        sx.add(p[lane]);
+        if constexpr (!POT) {
+            sy.add(p[plane + lane]);
+            sz.add(p[2 * plane + lane]);
+        }
+    }
+}
+
+template <bool POT>
 __global__ __launch_bounds__(64 * kSumSplit) void k_mutual_sum(const SumBlock* __restrict__ blocks,
                                                               const SimDesc* __restrict__ sims,
-                                                              const RowDesc* __restrict__ rows, int B,
+                                                              const RowDesc* __restrict__ rows, int B, int W,
                                                               const float* __restrict__ iside,
                                                               const float* __restrict__ jside,
-                                                              float4* __restrict__ gsum, int pot,
-                                                              const double* __restrict__ corr, int ntot,
+                                                              const float* __restrict__ jintra,
+                                                              float4* __restrict__ gsum,
+                                                              const float* __restrict__ corr, int ntot,
                                                               float4* __restrict__ force,
                                                               float4* __restrict__ mpos,
                                                               float4* __restrict__ mvel, SumMove mv) {
-    __shared__ double red[3][kSumSplit][64];
+    __shared__ float red[3][2][kSumSplit][64];  // per component: sum and compensation of each quarter
     const int lane = threadIdx.x & 63;
-    const int w = threadIdx.x >> 6;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform: the addresses below stay scalar
     const SumBlock blk = blocks[blockIdx.x];
     const SimDesc& sd = sims[blk.sim];
     const int q = blk.q0 + lane;
     const int row = blk.q0 / B;
-    const int o = q - row * B;
+    const int o0 = blk.q0 - row * B;  // lane 0's place in its row
     // the particle at padded place q, or -1 for a ghost
     int i = -1;
     if (q < sd.gfe) {
@@ -462,43 +687,70 @@ __global__ __launch_bounds__(64 * kSumSplit) void k_mutual_sum(const SumBlock* _
     } else if (q - sd.gfe < sd.end - sd.fe_end) {
         i = sd.fe_end + (q - sd.gfe);
     }
-    double sx = 0.0, sy = 0.0, sz = 0.0;
+    CompSum sx, sy, sz;
+    sx.zero();
+    sy.zero();
+    sz.zero();
     if (i >= 0) {
         const RowDesc rd = rows[sd.row0 + row];
-        for (int k = rd.nslot * w / kSumSplit; k < rd.nslot * (w + 1) / kSumSplit; ++k) {
-            const float* p = iside + (size_t)(rd.slot0 + k) * (3 * B) + o;
-            sx += (double)p[0];
-            if (!pot) {
-                sy += (double)p[B];
-                sz += (double)p[2 * B];
-            }
-        }
-        for (int r = row * w / kSumSplit; r < row * (w + 1) / kSumSplit; ++r) {
-            const long long len = sd.gn - (long long)(r + 1) * B;
-            const float* p = jside + sd.jbase + 3 * jside_row_offset(r, sd.gn, B) + (q - (r + 1) * B);
-            sx += (double)p[0];
-            if (!pot) {
-                sy += (double)p[len];
-                sz += (double)p[2 * len];
-            }
-        }
+        add_entries<POT>(
+            rd.nslot * w / kSumSplit, rd.nslot * (w + 1) / kSumSplit, lane,
+            [&](int k, const float*& p, long long& plane) {
+                p = iside + (size_t)(rd.slot0 + k) * (3 * B) + o0;
+                plane = B;
+            },
+            sx, sy, sz);
+        // the j-side entries of the earlier groups of W rows (W = 1: rows) ...
+        const int group = row / W, WB = W * B;
+        add_entries<POT>(
+            group * w / kSumSplit, group * (w + 1) / kSumSplit, lane,
+            [&](int r, const float*& p, long long& plane) {
+                plane = sd.gn - (long long)(r + 1) * WB;
+                p = jside + sd.jbase + 3 * jside_row_offset(r, sd.gn, WB) + (blk.q0 - (long long)(r + 1) * WB);
+            },
+            sx, sy, sz);
+        // ... and of the earlier rows of its own group
+        const int in_group = row - group * W;
+        const long long ilen = (long long)(W - 1) * B;
+        add_entries<POT>(
+            group * W + in_group * w / kSumSplit, group * W + in_group * (w + 1) / kSumSplit, lane,
+            [&](int r, const float*& p, long long& plane) {
+                p = jintra + sd.jibase + 3 * ilen * r + (blk.q0 - (long long)(r + 1) * B);
+                plane = ilen;
+            },
+            sx, sy, sz);
     }
-    red[0][w][lane] = sx;
-    red[1][w][lane] = sy;
-    red[2][w][lane] = sz;
+    red[0][0][w][lane] = sx.s;
+    red[0][1][w][lane] = sx.c;
+    red[1][0][w][lane] = sy.s;
+    red[1][1][w][lane] = sy.c;
+    red[2][0][w][lane] = sz.s;
+    red[2][1][w][lane] = sz.c;
     __syncthreads();
     if (w == 0 && i >= 0) {
-        double tx = 0.0, ty = 0.0, tz = 0.0;
+        // the quarters in order; their compensations join the total's
+        CompSum ax, ay, az;
+        ax.zero();
+        ay.zero();
+        az.zero();
 #pragma unroll
         for (int k = 0; k < kSumSplit; ++k) {
-            tx += red[0][k][lane];
-            ty += red[1][k][lane];
-            tz += red[2][k][lane];
+            ax.add(red[0][0][k][lane]);
+            ax.c += red[0][1][k][lane];
+            ay.add(red[1][0][k][lane]);
+            ay.c += red[1][1][k][lane];
+            az.add(red[2][0][k][lane]);
+            az.c += red[2][1][k][lane];
         }
         if (corr) {
-            const float fx = (float)((double)(float)tx + corr[i]);
-            const float fy = (float)((double)(float)ty + corr[ntot + i]);
-            const float fz = (float)((double)(float)tz + corr[2 * ntot + i]);
+            // k_finish's arithmetic: the correction's (sum, compensation)
+            // pair plus the all-pairs sum rounded to float, rounded once
+            CompSum cx{corr[i], corr[ntot + i]}, cy{corr[2 * ntot + i], corr[3 * ntot + i]},
+                cz{corr[4 * ntot + i], corr[5 * ntot + i]};
+            cx.add(ax.value());
+            cy.add(ay.value());
+            cz.add(az.value());
+            const float fx = cx.value(), fy = cy.value(), fz = cz.value();
             force[i].x = fx;
             force[i].y = fy;
             force[i].z = fz;
@@ -519,7 +771,7 @@ __global__ __launch_bounds__(64 * kSumSplit) void k_mutual_sum(const SumBlock* _
                 mpos[i] = p;
             }
         } else {
-            gsum[i] = make_float4((float)tx, (float)ty, (float)tz, 0.0f);
+            gsum[i] = make_float4(ax.value(), ay.value(), az.value(), 0.0f);
         }
     }
 }
@@ -654,9 +906,9 @@ __global__ void k_grid_gather(int ntot, const int* __restrict__ sorted, const fl
 // The contact correction of a particle, for every pair with r < 1 found
 // through the grid: the reference's exact contact term minus what the
 // all-pairs pass added for that pair.  Contact terms are rounded exactly as
-// the reference rounds them (float products of a float magnitude); only
-// their sum is carried in FP64, so cancellation between large contact forces
-// adds no summation error.
+// the reference rounds them (float products of a float magnitude); each
+// term and each gravity term subtracted goes into a compensated float sum,
+// so cancellation between large contact forces adds no summation error.
 //
 // The 27-cell walk is a chain of dependent loads (bucket bounds, then each
 // candidate).  It is cut into the nine rows of three cells (oz, oy) =
@@ -675,7 +927,7 @@ constexpr int kContactWaves = 4;  // wavefronts per workgroup
 __device__ __forceinline__ void contact_row(const SimDesc& sd, int t, int i, int row,
                                             const int* __restrict__ starts, const int* __restrict__ sorted,
                                             const float4* __restrict__ spos, const float4* __restrict__ svel,
-                                            int variant, double& ex, double& ey, double& ez) {
+                                            int variant, CompSum& ex, CompSum& ey, CompSum& ez) {
     const float4 pi = spos[t];
     const float4 vi = svel[t];
     const int ccx = cell_coord(pi.x), ccy = cell_coord(pi.y), ccz = cell_coord(pi.z);
@@ -711,9 +963,12 @@ __device__ __forceinline__ void contact_row(const SimDesc& sd, int t, int i, int
             // What pass 1 added for this pair.
             const float g = gravity_product(sd.law, test);
             const float sf = g * inv_r3_variant(variant, fast_r2(dpx, dpy, dpz));
-            ex += (double)(fm * dpx * invr) - (double)(sf * dpx);
-            ey += (double)(fm * dpy * invr) - (double)(sf * dpy);
-            ez += (double)(fm * dpz * invr) - (double)(sf * dpz);
+            ex.add(fm * dpx * invr);
+            ex.add(-(sf * dpx));
+            ey.add(fm * dpy * invr);
+            ey.add(-(sf * dpy));
+            ez.add(fm * dpz * invr);
+            ez.add(-(sf * dpz));
         }
     }
 }
@@ -735,7 +990,7 @@ __host__ __device__ constexpr int contact_blocks(int ntot) {
 struct ContactLane {
     int t, i, s;
     bool lead;
-    double ex, ey, ez;
+    CompSum ex, ey, ez;  // the correction as an unrounded (sum, compensation) pair
 };
 
 template <int LANES>
@@ -757,32 +1012,48 @@ __device__ __forceinline__ ContactLane contact_correction(int ntot, const SimDes
     c.i = c.t >= 0 ? sorted[c.t] : 0;
     c.s = c.t >= 0 ? find_sim(sims, nsims, c.i) : 0;
     const bool live = c.t >= 0 && sims[c.s].active;
-    c.ex = c.ey = c.ez = 0.0;
+    // each row summed from zero, then the nine rows' (sum, compensation)
+    // pairs added in row order (the same arithmetic on one lane or nine)
+    c.ex.zero();
+    c.ey.zero();
+    c.ez.zero();
     if constexpr (LANES == 1) {
         c.lead = live;
         if (live)
             for (int r = 0; r < kCellRows; ++r) {
-                double rx = 0.0, ry = 0.0, rz = 0.0;
+                CompSum rx, ry, rz;
+                rx.zero();
+                ry.zero();
+                rz.zero();
                 contact_row(sims[c.s], c.t, c.i, r, starts, sorted, spos, svel, variant, rx, ry, rz);
-                c.ex += rx;
-                c.ey += ry;
-                c.ez += rz;
+                c.ex.add_sum(rx);
+                c.ey.add_sum(ry);
+                c.ez.add_sum(rz);
             }
     } else {
-        __shared__ double part[kContactWaves][3][64];
-        double rx = 0.0, ry = 0.0, rz = 0.0;
+        __shared__ float part[kContactWaves][6][64];
+        CompSum rx, ry, rz;
+        rx.zero();
+        ry.zero();
+        rz.zero();
         if (live) contact_row(sims[c.s], c.t, c.i, row, starts, sorted, spos, svel, variant, rx, ry, rz);
-        part[w][0][lane] = rx;
-        part[w][1][lane] = ry;
-        part[w][2][lane] = rz;
+        part[w][0][lane] = rx.s;
+        part[w][1][lane] = rx.c;
+        part[w][2][lane] = ry.s;
+        part[w][3][lane] = ry.c;
+        part[w][4][lane] = rz.s;
+        part[w][5][lane] = rz.c;
         __syncthreads();
         c.lead = live && row == 0;
         if (c.lead) {
 #pragma unroll
             for (int r = 0; r < kCellRows; ++r) {
-                c.ex += part[w][0][lane + r];
-                c.ey += part[w][1][lane + r];
-                c.ez += part[w][2][lane + r];
+                c.ex.add(part[w][0][lane + r]);
+                c.ex.c += part[w][1][lane + r];
+                c.ey.add(part[w][2][lane + r]);
+                c.ey.c += part[w][3][lane + r];
+                c.ez.add(part[w][4][lane + r]);
+                c.ez.c += part[w][5][lane + r];
             }
         }
     }
@@ -805,31 +1076,29 @@ __global__ __launch_bounds__(64 * kContactWaves) void k_finish(float4* __restric
     const SimDesc& sd = sims[c.s];
     const int i = c.i;
     const int n = sd.end - sd.start;
-    float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+    // the correction's pair plus the all-pairs sums, rounded once
+    CompSum fx = c.ex, fy = c.ey, fz = c.ez;
     if (gsum && sd.gn > 0) {
         // mutual pass: already summed per particle
         const float4 gs = gsum[i];
-        fx = gs.x;
-        fy = gs.y;
-        fz = gs.z;
+        fx.add(gs.x);
+        fy.add(gs.y);
+        fz.add(gs.z);
     } else {
-        float kx = 0.0f, ky = 0.0f, kz = 0.0f;
         for (int ch = 0; ch < sd.nchunk; ++ch) {
             float4 p = partial[sd.partial_base + ch * n + (i - sd.start)];
-            kahan_add(fx, kx, p.x);
-            kahan_add(fy, ky, p.y);
-            kahan_add(fz, kz, p.z);
+            fx.add(p.x);
+            fy.add(p.y);
+            fz.add(p.z);
         }
-        fx -= kx;
-        fy -= ky;
-        fz -= kz;
     }
-    force[i].x = (float)((double)fx + c.ex);
-    force[i].y = (float)((double)fy + c.ey);
-    force[i].z = (float)((double)fz + c.ez);
+    force[i].x = fx.value();
+    force[i].y = fy.value();
+    force[i].z = fz.value();
 }
 
-// The contact corrections alone, into corr (three planes of ntot doubles,
+// The contact corrections alone, into corr (six planes of ntot floats: the
+// (sum, compensation) pair of x, then of y, then of z,
 // by particle index), for plans whose simulations all take the mutual pass:
 // it needs only the grid, so it runs beside the all-pairs pass, and
 // k_mutual_sum adds it to the sums -- the same arithmetic as k_finish.
@@ -839,12 +1108,15 @@ __global__ __launch_bounds__(64 * kContactWaves) void k_contact_corr(int ntot, c
                                                                      const int* __restrict__ sorted,
                                                                      const float4* __restrict__ spos,
                                                                      const float4* __restrict__ svel, int variant,
-                                                                     double* __restrict__ corr) {
+                                                                     float* __restrict__ corr) {
     const ContactLane c = contact_correction<LANES>(ntot, sims, nsims, starts, sorted, spos, svel, variant);
     if (!c.lead) return;
-    corr[c.i] = c.ex;
-    corr[ntot + c.i] = c.ey;
-    corr[2 * ntot + c.i] = c.ez;
+    corr[c.i] = c.ex.s;
+    corr[ntot + c.i] = c.ex.c;
+    corr[2 * ntot + c.i] = c.ey.s;
+    corr[3 * ntot + c.i] = c.ey.c;
+    corr[4 * ntot + c.i] = c.ez.s;
+    corr[5 * ntot + c.i] = c.ez.c;
 }
 
 // ---------------------------------------------------------------------------
@@ -862,10 +1134,12 @@ __global__ __launch_bounds__(64 * kContactWaves) void k_contact_corr(int ntot, c
 // simulations S times more wavefronts, each with a shorter chain.  With S = 1
 // the j come through wave-uniform scalar loads.  Every pair takes the
 // all-pairs pass's branch-free gravity term (float tiles of kDirectTile j
-// added in FP64); when some lane of a group has a pair with r^2 < 1 the group
-// also adds, for its pairs in contact, the exact contact term minus that
-// gravity term, with the arithmetic of k_finish.  The streams' sums are added
-// in FP64 in a fixed order, so the result depends on the simulation alone.
+// folded into a compensated float sum); when some lane of a group has a pair
+// with r^2 < 1 the group also adds, for its pairs in contact, the exact
+// contact term minus that gravity term, with the arithmetic of k_finish.  The
+// streams' sums are added in a compensated float sum in a fixed order, so the
+// result depends on the simulation alone.  (The potential variant sums in
+// FP64: it feeds the energy reduction.)
 // POT: the per-particle potential of k_finish_potential instead of the force.
 constexpr int kDirectWaves = 16;
 constexpr int kDirectTile = 256;  // j per float tile
@@ -887,6 +1161,15 @@ struct FusedMove {
     float damp1, damp2;
 };
 
+// FP64 sum with CompSum's interface: the potential pass feeds the energy
+// reduction, which stays FP64.
+struct F64Sum {
+    double s;
+    __device__ __forceinline__ void zero() { s = 0.0; }
+    __device__ __forceinline__ void add(double x) { s += x; }
+    __device__ __forceinline__ double value() const { return s; }
+};
+
 template <bool POT, int S, int MOVE>
 __global__ __launch_bounds__(64 * kDirectWaves) void k_direct(const float4* __restrict__ pos,
                                                               const float4* __restrict__ vel,
@@ -899,7 +1182,10 @@ __global__ __launch_bounds__(64 * kDirectWaves) void k_direct(const float4* __re
     constexpr int IB = 64 / S;               // i-particles per workgroup
     constexpr int NS = kDirectWaves * S;     // j-streams per particle
     constexpr int TG = kDirectTile / 4;      // groups per stream per float tile
-    __shared__ double red[3][kDirectWaves][64];
+    using Sum = typename std::conditional<POT, F64Sum, CompSum>::type;
+    using Val = typename std::conditional<POT, double, float>::type;
+    __shared__ Val red[3][kDirectWaves][64];
+    __shared__ float redc[POT ? 1 : 3][kDirectWaves][64];  // the force's compensations
     const int lane = threadIdx.x & 63;
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int stream = wv * S + lane / IB;  // wave-uniform when S == 1
@@ -914,7 +1200,10 @@ __global__ __launch_bounds__(64 * kDirectWaves) void k_direct(const float4* __re
 
     float4 vi = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if constexpr (!POT) vi = vel[ic];
-    double sx = 0.0, sy = 0.0, sz = 0.0;  // this wavefront's gravity plus contact corrections
+    Sum sx, sy, sz;  // this wavefront's gravity plus contact corrections
+    sx.zero();
+    sy.zero();
+    sz.zero();
     for (int seg = 0; seg < 2; ++seg) {
         const bool jfe = seg == 0;  // iron j first, then silicate j
         const int a = jfe ? start : fe_end;
@@ -923,8 +1212,14 @@ __global__ __launch_bounds__(64 * kDirectWaves) void k_direct(const float4* __re
         if (stream >= ngroups) continue;
         const int test = (int)ife + (int)jfe;
         const float g = gravity_product(law, test);
-        double ux = 0.0, uy = 0.0, uz = 0.0;  // float tiles of kDirectTile j, folded in FP64
-        double cx = 0.0, cy = 0.0, cz = 0.0;
+        Sum ux, uy, uz;  // float tiles of kDirectTile j, folded (compensated; FP64 for the potential)
+        Sum cx, cy, cz;
+        ux.zero();
+        uy.zero();
+        uz.zero();
+        cx.zero();
+        cy.zero();
+        cz.zero();
         // The exact contact term of pair (i, j) minus the gravity term s*dp
         // the tile sum took for it (k_finish's, resp. k_finish_potential's,
         // test and arithmetic).
@@ -942,9 +1237,12 @@ __global__ __launch_bounds__(64 * kDirectWaves) void k_direct(const float4* __re
                     const float dot = dx * dvx + dy * dvy + dz * dvz;
                     const float fm = contact_pair_mag(law, test, r, r2, dot);
                     const float sf = g * s;
-                    cx += (double)(fm * dx * invr) - (double)(sf * dx);
-                    cy += (double)(fm * dy * invr) - (double)(sf * dy);
-                    cz += (double)(fm * dz * invr) - (double)(sf * dz);
+                    cx.add(fm * dx * invr);
+                    cx.add(-(sf * dx));
+                    cy.add(fm * dy * invr);
+                    cy.add(-(sf * dy));
+                    cz.add(fm * dz * invr);
+                    cz.add(-(sf * dz));
                 }
             } else {
                 const double ddx = (double)pj.x - pi.x, ddy = (double)pj.y - pi.y, ddz = (double)pj.z - pi.z;
@@ -953,7 +1251,7 @@ __global__ __launch_bounds__(64 * kDirectWaves) void k_direct(const float4* __re
                     const double G = gravity_product(law, test);
                     const double K = approach_k(law, test);
                     const double exact = G * (r - 2.0) + (K / 3.0) * (r - 1.0) * (r - 1.0) * (r + 2.0);
-                    cx += exact + G * (double)s;
+                    cx.add(exact + G * (double)s);
                 }
             }
         };
@@ -1013,34 +1311,55 @@ __global__ __launch_bounds__(64 * kDirectWaves) void k_direct(const float4* __re
 #pragma unroll
                 for (int u = 0; u < 4; ++u) q[u] = nq[u];
             }
-            ux += (double)tx;
-            uy += (double)ty;
-            uz += (double)tz;
+            ux.add(tx);
+            uy.add(ty);
+            uz.add(tz);
         }
         if constexpr (!POT) {
-            sx += (double)g * ux + cx;
-            sy += (double)g * uy + cy;
-            sz += (double)g * uz + cz;
+            sx.add_product(g, ux.s);
+            sx.c += g * ux.c;
+            sx.add_sum(cx);
+            sy.add_product(g, uy.s);
+            sy.c += g * uy.c;
+            sy.add_sum(cy);
+            sz.add_product(g, uz.s);
+            sz.c += g * uz.c;
+            sz.add_sum(cz);
         } else {
-            sx += -(double)g * ux + cx;
+            sx.add(-(double)g * ux.value() + cx.value());
         }
     }
-    red[0][wv][lane] = sx;
-    red[1][wv][lane] = sy;
-    red[2][wv][lane] = sz;
+    if constexpr (!POT) {
+        red[0][wv][lane] = sx.s;
+        red[1][wv][lane] = sy.s;
+        red[2][wv][lane] = sz.s;
+        redc[0][wv][lane] = sx.c;
+        redc[1][wv][lane] = sy.c;
+        redc[2][wv][lane] = sz.c;
+    } else {
+        red[0][wv][lane] = sx.value();
+    }
     __syncthreads();
     if (wv == 0 && lane < IB && live) {
-        double fx = 0.0, fy = 0.0, fz = 0.0;
+        Sum fx, fy, fz;
+        fx.zero();
+        fy.zero();
+        fz.zero();
 #pragma unroll
         for (int w = 0; w < kDirectWaves; ++w)
 #pragma unroll
             for (int q = 0; q < S; ++q) {
-                fx += red[0][w][q * IB + lane];
-                fy += red[1][w][q * IB + lane];
-                fz += red[2][w][q * IB + lane];
+                fx.add(red[0][w][q * IB + lane]);
+                if constexpr (!POT) {
+                    fx.c += redc[0][w][q * IB + lane];
+                    fy.add(red[1][w][q * IB + lane]);
+                    fy.c += redc[1][w][q * IB + lane];
+                    fz.add(red[2][w][q * IB + lane]);
+                    fz.c += redc[2][w][q * IB + lane];
+                }
             }
         if constexpr (!POT) {
-            const float f0 = (float)fx, f1 = (float)fy, f2 = (float)fz;
+            const float f0 = fx.value(), f1 = fy.value(), f2 = fz.value();
             force[i].x = f0;
             force[i].y = f1;
             force[i].z = f2;
@@ -1079,7 +1398,7 @@ __global__ __launch_bounds__(64 * kDirectWaves) void k_direct(const float4* __re
                 mv.pout[i] = p;
             }
         } else {
-            pe[i] = 0.5 * fx;
+            pe[i] = 0.5 * fx.value();
         }
     }
 }
@@ -1479,5 +1798,13 @@ GI_MUTUAL_INST(2, true)
 GI_MUTUAL_INST(4, true)
 GI_MUTUAL_INST(8, true)
 #undef GI_MUTUAL_INST
+#define GI_MUTUAL_GRP_INST(R, POT)                                                                              \
+    template __global__ void k_mutual_grp<R, POT>(const float4*, const GroupTask*, int, const MutualTask*,     \
+                                                  const SimDesc*, const RowDesc*, float*, float*, float*);
+GI_MUTUAL_GRP_INST(4, false)
+GI_MUTUAL_GRP_INST(8, false)
+GI_MUTUAL_GRP_INST(4, true)
+GI_MUTUAL_GRP_INST(8, true)
+#undef GI_MUTUAL_GRP_INST
 
 }  // namespace gi

@@ -6,7 +6,10 @@
 //              pair once for both particles, rows of 64*R, rsq cubed);
 //              3 = direct pass (small simulations: the whole force in one
 //              launch; R = lanes per i-particle, 1, 2 or 4 (8 acts as 4),
-//              64 / R i-particles per workgroup; the sizes unused)
+//              64 / R i-particles per workgroup; the sizes unused);
+//              4 = mutual pass with row groups (a workgroup's four
+//              wavefronts hold four rows and add their j-side sums in LDS
+//              before writing them: a quarter of the j-side slab; R = 4 or 8)
 // chunk_len    j-particles per task (0 = derive from target_tasks)
 // target_tasks wavefront tasks to aim for (several per SIMD of the 256 CUs x
 //              4 SIMDs so the tail of the launch stays short)
@@ -76,7 +79,8 @@ inline const std::vector<TuningRow>& tuning_table() {
             TuningRow r;
             if (std::sscanf(line, "%lld %d %d %lld %lld %lld", &r.n_max, &r.t.R, &r.t.variant, &r.t.chunk_len,
                             &r.t.target_tasks, &r.t.min_chunk) == 6 &&
-                (r.t.R == 1 || r.t.R == 2 || r.t.R == 4 || r.t.R == 8) && r.t.variant >= 0 && r.t.variant <= 3 &&
+                (r.t.R == 1 || r.t.R == 2 || r.t.R == 4 || r.t.R == 8) && r.t.variant >= 0 && r.t.variant <= 4 &&
+                (r.t.variant != 4 || r.t.R >= 4) &&
                 r.t.chunk_len >= 0 && r.t.target_tasks >= 0 &&
                 (r.t.target_tasks > 0 || r.t.chunk_len > 0 || r.t.variant == 3))
                 rows.push_back(r);
@@ -121,7 +125,8 @@ inline BatchedTiling batched_tiling() {
             long long L;
             int R, V = 0;
             const int got = std::sscanf(line, "B %lld %d %d", &L, &R, &V);
-            if (got >= 2 && L >= 64 && (R == 1 || R == 2 || R == 4 || R == 8) && V >= 0 && V <= 2)
+            if (got >= 2 && L >= 64 && (R == 1 || R == 2 || R == 4 || R == 8) &&
+                (V == 0 || V == 1 || V == 2 || (V == 4 && R >= 4)))
                 from_file = BatchedTiling{L, R, V};
         }
         std::fclose(f);

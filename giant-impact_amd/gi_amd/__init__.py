@@ -487,8 +487,8 @@ def plan_info() -> dict:
     """The plan of the last drop-in force call on the current device."""
     m, r, t, L = C.c_int(), C.c_int(), C.c_longlong(), C.c_longlong()
     call("gi_plan_info", C.byref(m), C.byref(r), C.byref(t), C.byref(L))
-    return {"mutual": m.value == 1, "direct": m.value == 2, "r": r.value, "pair_tasks": t.value,
-            "chunk_len": L.value, "kernel": ("k_gravity", "k_mutual", "k_direct")[m.value]}
+    return {"mutual": m.value in (1, 3), "direct": m.value == 2, "r": r.value, "pair_tasks": t.value,
+            "chunk_len": L.value, "kernel": ("k_gravity", "k_mutual", "k_direct", "k_mutual_grp")[m.value]}
 
 
 def sync() -> None:

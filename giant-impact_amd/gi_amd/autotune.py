@@ -4,8 +4,9 @@ For each size class it times the whole force evaluation (HIP events, via
 gi_kernel_timing("force"): contact grid, all-pairs pass and contact
 correction, or the direct pass alone) over a grid of tilings -- the
 one-sided pass with R (i-particles per lane), the r^-3 variant and the
-number of wavefront tasks, the mutual pass (variant 2) with R (rows of 64*R)
-and its chunk length, and for the small classes the direct pass (variant 3)
+number of wavefront tasks, the mutual pass (variant 2) and its row-group form
+(variant 4) with R (rows of 64*R) and their chunk length, and for the small
+classes the direct pass (variant 3)
 -- and writes the fastest to tuning/gfx950.txt, which libgiant_impact.so
 reads on first use:
 
@@ -37,6 +38,7 @@ CLASSES = [
     (24576, 15360, 5120),
     (49152, 24576, 8192),
     (98304, 49152, 16384),
+    (196608, 98304, 32768),
     (393216, 196608, 65536),
     (2097152, 786432, 262144),
 ]
@@ -81,7 +83,7 @@ def tune_batched(quick: bool):
     best = None
     Ls = [1024, 2048, 4096, 8192, 16384] if not quick else [4096]
     cands = [(R, L, 0) for R, L in itertools.product([1, 2], Ls)]
-    cands += [(R, L, 2) for R, L in itertools.product([4, 8], [256, 384, 512, 768, 1024, 2048])]
+    cands += [(R, L, v) for R, L, v in itertools.product([4, 8], [256, 384, 512, 768, 1024, 2048], [2, 4])]
     for R, L, v in cands:
         call("gi_set_batched_tiling", R, L, v)
         P.upload(pos)
@@ -143,7 +145,7 @@ def main(argv=None) -> int:
         mc = 128 if n < 4096 else 512
         # (R, variant, chunk_len, target_tasks, min_chunk)
         cands = [(R, v, 0, tt, mc) for R, v, tt in itertools.product(Rs, variants, targets)]
-        cands += [(R, 2, L, 0, 64) for R, L in itertools.product([4, 8], mutual_Ls)]
+        cands += [(R, v, L, 0, 64) for R, L, v in itertools.product([4, 8], mutual_Ls, [2, 4])]
         if n_max <= DIRECT_MAX_CLASS:
             cands += [(S, 3, 0, 0, 0) for S in (1, 2, 4)]  # R = lanes per i-particle
         best = None

@@ -158,9 +158,9 @@ int gi_sync(void);
 /* Free and total device memory in bytes (hipMemGetInfo). */
 int gi_mem_info(size_t *free_bytes, size_t *total_bytes);
 /* The plan of the last drop-in force call on the current device: mutual = 1
- * when it uses the mutual all-pairs pass (k_mutual) for some simulation, 2
- * when it uses the direct pass (k_direct), 0 for the one-sided pass
- * (k_gravity); r = i-particles per lane, pair_tasks = wavefront tasks of the
+ * when it uses the mutual all-pairs pass (k_mutual) for some simulation, 3
+ * for its row-group form (k_mutual_grp), 2 when it uses the direct pass
+ * (k_direct), 0 for the one-sided pass (k_gravity); r = i-particles per lane, pair_tasks = wavefront tasks of the
  * all-pairs kernel(s) (workgroups of the direct pass), chunk_len = j per task
  * of the mutual pass.  Zeros before the first call.  Any pointer may be NULL. */
 int gi_plan_info(int *mutual, int *r, long long *pair_tasks, long long *chunk_len);
@@ -375,6 +375,15 @@ int gi_tiling_for(long long n, int *r, int *variant, long long *chunk_len, long 
  * default; takes effect at the next re-plan (a new ensemble or a change of the
  * active set). */
 int gi_set_batched_tiling(int r, long long chunk_len, int variant);
+
+/* Host only, no device needed: build the mutual pass's plan (variant 2, or
+ * 4 with row groups) for simulations of n[s] particles (nfe[s] irons) with
+ * r i-particles per lane and chunk_len j per task (batched = the batched
+ * engine's planning), and check every index its kernels derive from it
+ * against the buffers it sizes.  0 when all are in bounds, else GI_E_STATE
+ * with the first violation in gi_last_error_string().  No reference
+ * counterpart. */
+int gi_plan_check(int n_sims, const int *n, const int *nfe, int r, int variant, long long chunk_len, int batched);
 
 /* Wavefront tasks per all-pairs kernel dispatch: a longer task list is
  * launched as consecutive dispatches of at most max_tasks tasks on the same

@@ -26,7 +26,10 @@ conditioned error has no such escape.
 Particles with a pair whose branch decision lies within 1e-6 of a threshold
 (r = 1, a shell break, dp.dv = 1) are excluded: there a last-bit rounding
 difference legitimately selects another branch of the reference's law.  At
-most 0.1% of the rows may be excluded that way.
+most 0.1% of the rows may be excluded that way, and never fewer than 8 rows
+(four pairs, each excusing both its particles) are allowed: in a contact
+state of 2,048 particles one or two pairs within the margin are a matter of
+chance (the receding test's margin scales with |dp| |dv|).
 """
 from __future__ import annotations
 
@@ -67,10 +70,11 @@ def force_metrics(F, F64, cond, near, Fref=None) -> dict:
 
 
 NEAR_MAX_FRACTION = 1e-3  # rows a last-bit branch difference may excuse
+NEAR_MIN_ROWS = 8
 
 
 def assert_force_parity(met: dict, check_ref: bool = True) -> None:
-    assert met["near_threshold"] <= NEAR_MAX_FRACTION * met["n"], met
+    assert met["near_threshold"] <= max(NEAR_MAX_FRACTION * met["n"], NEAR_MIN_ROWS), met
     assert met["cond_max"] <= COND_TOL, met
     if "rel_excess_max" in met:
         assert met["rel_excess_max"] <= REL_TOL, met

@@ -309,25 +309,63 @@ def test_conserved_quantities_match_fp64_oracle(gpu, oracle):
 
 
 def test_conserved_drift_tracks_the_reference_float_path(gpu, oracle):
-    """300 steps of the N = 2,048 impact on the GPU and with the oracle's float
-    restatement of the reference kernels: the FP64 energy change (dissipation
-    by the reduced receding stiffness) agrees, and the angular and linear
-    momentum drifts stay at the reference path's float-rounding level."""
-    pos, vel, fc, mc, _ = gpu.make_impact(1536, 512, seed=20261202, angle_deg=45.0, speed_vesc=1.2)
+    """Drift at N = 2,048: 16 seeded impacts, 1,000 steps each on the GPU and
+    with the oracle's float restatement of the reference kernels.  The FP64
+    energy change (dissipation by the reduced receding stiffness, ~50% here)
+    agrees per seed, and the median linear-momentum drift over the seeds is
+    no worse than the reference float path's.  The angular-momentum drift of
+    both sits at the float rounding floor of the shared kick-drift move
+    (per seed 1.5e-8 to 9e-8 on either path, a 16-seed median uncertain by
+    about 20%): their medians agree within 25%.  Where the force's own
+    rounding dominates (N >= 6,575), the GPU drifts 10 to 1,000 times less
+    than the reference float path (the next test, and the drift study)."""
+    steps = 1000
+    dLg, dLr, dPg, dPr = [], [], [], []
+    for k in range(16):
+        pos, vel, fc, mc, _ = gpu.make_impact(1536, 512, seed=20261400 + k, angle_deg=45.0, speed_vesc=1.2)
+        n = len(pos)
+        c0 = oracle.conserved_f64(pos, vel, fc)
+        P, V, F = _upload(gpu, pos, vel)
+        gpu.call("gi_impact_steps", P.ptr, V.ptr, F.ptr, n, C.byref(fc), C.byref(mc), steps, 0, None, None, None)
+        gpu.sync()
+        cg = oracle.conserved_f64(P.download(), V.download(), fc)
+        po, vo, _ = oracle.impact_steps_f32(pos, vel, fc, mc, steps)
+        cr = oracle.conserved_f64(po, vo, fc)
+        dEg, dEr = (cg[2] - c0[2]) / abs(c0[2]), (cr[2] - c0[2]) / abs(c0[2])
+        assert abs(dEg - dEr) <= 0.01 * abs(dEr), (k, dEg, dEr)
+        L0 = np.linalg.norm(c0[6:9])
+        dLg.append(np.linalg.norm(cg[6:9] - c0[6:9]) / L0)
+        dLr.append(np.linalg.norm(cr[6:9] - c0[6:9]) / L0)
+        dPg.append(np.linalg.norm(cg[3:6] - c0[3:6]))
+        dPr.append(np.linalg.norm(cr[3:6] - c0[3:6]))
+    print(f"median dL gpu {np.median(dLg):.3e} ref {np.median(dLr):.3e}; "
+          f"median dP gpu {np.median(dPg):.3e} ref {np.median(dPr):.3e}")
+    assert np.median(dLg) <= 1.25 * np.median(dLr)
+    assert np.median(dPg) <= np.median(dPr)
+
+
+def test_conserved_drift_beats_the_reference_float_path_at_13k(gpu, oracle):
+    """N = 13,255 (a simulation of the 64-impact ensemble), 200 steps through
+    the batched engine and with the reference float path: the reference's
+    sequential float sum over j drifts an order of magnitude more in angular
+    momentum (measured 2.8e-8 against 2.9e-7) and far more in linear
+    momentum."""
+    cfg = gpu.ensemble_configs(3, seed=20261205)[1]
+    pos, vel, fc, mc, _ = gpu.make_impact(**cfg)
     n = len(pos)
-    steps = 300
+    print(f"n={n}", flush=True)
     c0 = oracle.conserved_f64(pos, vel, fc)
-    P, V, F = _upload(gpu, pos, vel)
-    gpu.call("gi_impact_steps", P.ptr, V.ptr, F.ptr, n, C.byref(fc), C.byref(mc), steps, 0, None, None, None)
+    P, V = gpu.DeviceArray(n, pos), gpu.DeviceArray(n, vel)
+    ens = gpu.Ensemble(P, V, np.array([n], np.int32), (gpu.ForceCollisionConstants * 1)(fc),
+                       (gpu.MoveCollisionConstants * 1)(mc))
+    ens.steps(200)
     gpu.sync()
     cg = oracle.conserved_f64(P.download(), V.download(), fc)
-    po, vo, _ = oracle.impact_steps_f32(pos, vel, fc, mc, steps)
+    ens.close()
+    po, vo, _ = oracle.impact_steps_f32(pos, vel, fc, mc, 200)
     cr = oracle.conserved_f64(po, vo, fc)
-    dEg, dEr = (cg[2] - c0[2]) / abs(c0[2]), (cr[2] - c0[2]) / abs(c0[2])
     L0 = np.linalg.norm(c0[6:9])
     dLg, dLr = np.linalg.norm(cg[6:9] - c0[6:9]) / L0, np.linalg.norm(cr[6:9] - c0[6:9]) / L0
     dPg, dPr = np.linalg.norm(cg[3:6] - c0[3:6]), np.linalg.norm(cr[3:6] - c0[3:6])
-    print(f"dE gpu {dEg:.6e} ref {dEr:.6e}; dL gpu {dLg:.3e} ref {dLr:.3e}; dP gpu {dPg:.3e} ref {dPr:.3e}")
-    assert abs(dEg - dEr) <= 0.02 * abs(dEr) + 1e-5
-    assert dLg <= max(10 * dLr, 1e-6)
-    assert dPg <= max(10 * dPr, 1e-6 * abs(c0[0]) ** 0.5)
+    print(f"dL gpu {dLg:.3e} ref {dLr:.3e}; dP gpu {dPg:.3e} ref {dPr:.3e}")
+    assert dLg <= dLr and dPg <= dPr

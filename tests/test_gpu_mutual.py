@@ -197,11 +197,13 @@ def test_mutual_batched_is_bitwise_the_same_alone_and_in_the_mix(gpu, batched_mu
     ens.close()
 
 
-def test_mixed_plan_too_large_simulation_takes_the_one_sided_pass(gpu, batched_mutual):
-    """A simulation whose j-side slab would pass the per-simulation cap
-    (N = 1.3 M) takes the one-sided pass inside the same batched launch as two
-    small mutual-pass simulations; each small one stays bitwise the same as
-    alone, and the large one agrees with its single-simulation path."""
+def test_mixed_plan_too_large_simulation_takes_the_one_sided_pass(gpu, batched_mutual, tiling):
+    """A simulation whose j-side slab would pass the per-simulation cap of the
+    mutual pass with one row per slab group (N = 1.3 M) takes the one-sided
+    pass inside the same batched launch as two small mutual-pass simulations;
+    each small one stays bitwise the same as alone, and the large one agrees
+    with its single-simulation path, which falls back too with that pass and
+    keeps the mutual pass in its row-group form (a quarter of the slab)."""
     cfg = [dict(n_target=3000, n_impactor=1000, seed=1, angle_deg=40.0, speed_vesc=1.1, target_spin=0.2),
            dict(n_target=975000, n_impactor=325000, seed=2, angle_deg=45.0, speed_vesc=1.1, target_spin=0.0),
            dict(n_target=4000, n_impactor=1200, seed=3, angle_deg=50.0, speed_vesc=1.2, target_spin=0.4)]
@@ -220,13 +222,16 @@ def test_mixed_plan_too_large_simulation_takes_the_one_sided_pass(gpu, batched_m
         assert np.array_equal(p_all[a:b], p1) and np.array_equal(v_all[a:b], v1), s
     a, b = off[1], off[2]
     nb = b - a
-    Pb, Vb, Fb = gpu.DeviceArray(nb, pos[a:b]), gpu.DeviceArray(nb, vel[a:b]), gpu.DeviceArray(nb, f4(nb))
-    gpu.impact_steps(Pb, Vb, Fb, nb, fc[1], mc[1], 1)
-    gpu.sync()
-    assert not gpu.plan_info()["mutual"]  # the single-simulation path falls back too
-    dv = v_all[a:b, :3].astype(np.float64) - Vb.download()[:, :3]
     kick = v_all[a:b, :3].astype(np.float64) - vel[a:b, :3]
-    assert np.abs(dv).max() <= 1e-5 * np.abs(kick).max()
+    for variant, mutual in ((2, False), (4, True)):
+        tiling(8, variant, 2048)
+        Pb, Vb, Fb = gpu.DeviceArray(nb, pos[a:b]), gpu.DeviceArray(nb, vel[a:b]), gpu.DeviceArray(nb, f4(nb))
+        gpu.impact_steps(Pb, Vb, Fb, nb, fc[1], mc[1], 1)
+        gpu.sync()
+        assert gpu.plan_info()["mutual"] == mutual, variant
+        dv = v_all[a:b, :3].astype(np.float64) - Vb.download()[:, :3]
+        assert np.abs(dv).max() <= 1e-5 * np.abs(kick).max(), variant
+        del Pb, Vb, Fb
 
 
 def test_large_simulation_mutual_pass_against_fp64(gpu, oracle):

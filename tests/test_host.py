@@ -138,7 +138,8 @@ def test_invalid_arguments_are_rejected_before_touching_a_device(gi):
     assert lib.gi_force_separate(vp(1), vp(1), vp(1), 10, C.byref(fs), None) == -1
     assert lib.gi_move_collision(vp(1), vp(1), vp(1), -3, C.byref(mc), None) == -1
     assert lib.gi_check_block_multiple(1000) == -2 and lib.gi_check_block_multiple(1024) == 0
-    assert lib.gi_set_tiling(3, 0, 0, 0, 0) == -1 and lib.gi_set_tiling(4, 4, 0, 0, 0) == -1
+    assert lib.gi_set_tiling(3, 0, 0, 0, 0) == -1 and lib.gi_set_tiling(4, 5, 0, 0, 0) == -1
+    assert lib.gi_set_tiling(2, 4, 0, 0, 0) == -1  # the row-group form needs rows of 256 or 512
     assert lib.gi_set_tiling(0, 0, 0, 0, 0) == 0
     assert lib.gi_set_dispatch_limit(-1) == -1 and lib.gi_set_dispatch_limit(0) == 0
     assert lib.gi_impact_steps(vp(1), vp(1), vp(1), 10, C.byref(fc), C.byref(mc), 5, 1, None, None, None) == -1
@@ -239,3 +240,25 @@ def test_bench_refuses_more_than_one_gpu():
                        timeout=120)
     assert r.returncode == 2
     assert "one GPU" in r.stderr and r.stdout == ""
+
+
+def test_mutual_plans_stay_in_bounds(gi):
+    """Every index the mutual pass's kernels derive from a plan (k_mutual or
+    the row-group kernel, then the per-particle sums) lies inside the buffers
+    the plan sizes: ragged, one-material and multi-simulation cases at every
+    row size and chunk length (host only, gi_plan_check)."""
+    lib = gi.load()
+    rng = np.random.default_rng(7)
+    cases = [([1], [0]), ([2], [1]), ([65], [0]), ([257], [100]), ([1100], [512]), ([2100], [1600]),
+             ([5000], [3000]), ([4097], [1229]), ([262144], [78643])]
+    for _ in range(40):
+        k = int(rng.integers(1, 6))
+        ns = [int(rng.integers(1, 30000)) for _ in range(k)]
+        cases.append((ns, [int(rng.integers(0, x + 1)) for x in ns]))
+    for ns, nfes in cases:
+        a, b = (C.c_int * len(ns))(*ns), (C.c_int * len(ns))(*nfes)
+        for r, v in ((8, 4), (4, 4), (8, 2), (4, 2), (1, 2)):
+            for L, batched in ((64, 0), (1024, 1)):
+                rc = lib.gi_plan_check(len(ns), a, b, r, v, C.c_longlong(L), batched)
+                assert rc == 0, (ns, nfes, r, v, L, lib.gi_last_error_string())
+    assert lib.gi_plan_check(1, (C.c_int * 1)(10), (C.c_int * 1)(3), 2, 4, C.c_longlong(64), 0) == -1  # R >= 4
