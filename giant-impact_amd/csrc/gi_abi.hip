@@ -119,6 +119,8 @@ struct Plan {
     // beside it (k_contact_corr) and k_mutual_sum writes the force
     bool split_finish = false;
     bool overlap_allowed = true;  // gi_set_contact_overlap when planned
+    int contact_lanes = 1;        // lanes per particle of the contact walk (1 or 9)
+    int contact_walk = 0;         // gi_set_contact_walk when planned
     // direct pass (tiling.variant == 3): one workgroup per 64 / direct_lanes
     // i-particles
     bool direct = false;
@@ -140,7 +142,7 @@ struct Workspace {
     DevBuf<DirectBlock> dblocks;
     DevBuf<int> counts, starts, sorted, cellhash, block_sums;
     DevBuf<float4> spos, svel;  // positions / velocities in bucket order
-    DevBuf<float> corr;         // contact corrections ((sum, compensation) x 3 planes of ntot)
+    DevBuf<float> corr;         // contact corrections: per particle (sum, compensation) of x, y, z
     DevBuf<GravTask> tasks;
     DevBuf<SimDesc> sims;
     DevBuf<int> block_sim, block_first, sim_first_block;
@@ -465,6 +467,11 @@ static int plan_mutual(std::vector<SimDesc>& sims, const Tiling& tl, bool batche
 // blocks.  batched: every simulation uses the fixed tiling of the batched
 // engine, so its sums are the same alone and in any batch.
 static std::atomic<int> g_contact_overlap{1};  // see gi_set_contact_overlap
+static std::atomic<int> g_contact_walk{0};     // see gi_set_contact_walk
+
+// The contact walk for plans of up to this many particles uses nine lanes per
+// particle: their all-pairs pass is too short to hide a one-lane walk.
+constexpr int kContactRowLanesMaxN = 49152;
 
 static int build_plan(std::vector<SimDesc> sims, const Tiling& tl, bool batched, Plan* out) {
     Plan p;
@@ -563,6 +570,8 @@ static int build_plan(std::vector<SimDesc> sims, const Tiling& tl, bool batched,
         p.sim_first_block.push_back((int)p.block_sim.size());
     }
     p.overlap_allowed = g_contact_overlap.load() != 0;
+    p.contact_walk = g_contact_walk.load();
+    p.contact_lanes = p.contact_walk ? p.contact_walk : (p.ntot <= kContactRowLanesMaxN ? 9 : 1);
     p.split_finish = p.mutual && p.tasks.empty() && !p.direct && p.overlap_allowed;
     p.sims = std::move(sims);
     *out = std::move(p);
@@ -703,7 +712,8 @@ static int ensure_plan(Workspace* ws, const std::vector<SimDesc>& sims, hipStrea
         tl = Tiling{bt.R, bt.variant, bt.chunk_len, 0, 0};
     }
     if (ws->plan_valid && same_tiling(ws->plan.tiling, tl) && same_sims(ws->plan.caller_sims, sims) &&
-        ws->plan.overlap_allowed == (g_contact_overlap.load() != 0))
+        ws->plan.overlap_allowed == (g_contact_overlap.load() != 0) &&
+        ws->plan.contact_walk == g_contact_walk.load())
         return 0;
     Plan p;
     int rc = build_plan(sims, tl, ws->batched, &p);
@@ -983,10 +993,6 @@ static int launch_direct(DevCtx* ctx, Workspace* ws, const float4* pos, const fl
     return 0;
 }
 
-// The contact walk for plans of up to this many particles uses nine lanes per
-// particle: their all-pairs pass is too short to hide a one-lane walk.
-constexpr int kContactRowLanesMaxN = 49152;
-
 template <int LANES>
 static void launch_contact_lanes(const Workspace* ws, float4* force, hipStream_t st) {
     const Plan& p = ws->plan;
@@ -1003,7 +1009,7 @@ static void launch_contact_lanes(const Workspace* ws, float4* force, hipStream_t
 // With force: k_finish (sums + corrections into force); without: the
 // corrections alone into ws->corr (k_contact_corr).
 static void launch_contact(const Workspace* ws, float4* force, hipStream_t st) {
-    if (ws->plan.ntot <= kContactRowLanesMaxN)
+    if (ws->plan.contact_lanes == kCellRows)
         launch_contact_lanes<kCellRows>(ws, force, st);
     else
         launch_contact_lanes<1>(ws, force, st);
@@ -1522,6 +1528,13 @@ int gi_set_batched_tiling(int r, long long chunk_len, int variant) {
 
 int gi_set_contact_overlap(int enable) {
     g_contact_overlap.store(enable ? 1 : 0);
+    return 0;
+}
+
+int gi_set_contact_walk(int lanes) {
+    if (lanes != 0 && lanes != 1 && lanes != kCellRows)
+        return set_error(GI_E_INVALID_ARG, "gi_set_contact_walk: lanes in {0 (by plan size), 1, 9}");
+    g_contact_walk.store(lanes);
     return 0;
 }
 

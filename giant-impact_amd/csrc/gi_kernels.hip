@@ -745,8 +745,9 @@ __global__ __launch_bounds__(64 * kSumSplit) void k_mutual_sum(const SumBlock* _
         if (corr) {
             // k_finish's arithmetic: the correction's (sum, compensation)
             // pair plus the all-pairs sum rounded to float, rounded once
-            CompSum cx{corr[i], corr[ntot + i]}, cy{corr[2 * ntot + i], corr[3 * ntot + i]},
-                cz{corr[4 * ntot + i], corr[5 * ntot + i]};
+            const float2* rec = reinterpret_cast<const float2*>(corr + 6 * (size_t)i);
+            const float2 rx = rec[0], ry = rec[1], rz = rec[2];
+            CompSum cx{rx.x, rx.y}, cy{ry.x, ry.y}, cz{rz.x, rz.y};
             cx.add(ax.value());
             cy.add(ay.value());
             cz.add(az.value());
@@ -960,15 +961,13 @@ __device__ __forceinline__ void contact_row(const SimDesc& sd, int t, int i, int
             const float dvx = vj.x - vi.x, dvy = vj.y - vi.y, dvz = vj.z - vi.z;
             const float dot = dpx * dvx + dpy * dvy + dpz * dvz;
             const float fm = contact_pair_mag(sd.law, test, r, r2, dot);
-            // What pass 1 added for this pair.
+            // The contact term minus what pass 1 added for this pair (for
+            // r < 1 a bounded placeholder: |sf dp| < g), one addition each.
             const float g = gravity_product(sd.law, test);
             const float sf = g * inv_r3_variant(variant, fast_r2(dpx, dpy, dpz));
-            ex.add(fm * dpx * invr);
-            ex.add(-(sf * dpx));
-            ey.add(fm * dpy * invr);
-            ey.add(-(sf * dpy));
-            ez.add(fm * dpz * invr);
-            ez.add(-(sf * dpz));
+            ex.add(fm * dpx * invr - sf * dpx);
+            ey.add(fm * dpy * invr - sf * dpy);
+            ez.add(fm * dpz * invr - sf * dpz);
         }
     }
 }
@@ -1097,9 +1096,10 @@ __global__ __launch_bounds__(64 * kContactWaves) void k_finish(float4* __restric
     force[i].z = fz.value();
 }
 
-// The contact corrections alone, into corr (six planes of ntot floats: the
-// (sum, compensation) pair of x, then of y, then of z,
-// by particle index), for plans whose simulations all take the mutual pass:
+// The contact corrections alone, into corr (per particle, by index, one
+// 24-byte record: the (sum, compensation) pairs of x, y and z -- the lanes
+// walk in bucket order, so one record per lane keeps each store to one or
+// two cache lines), for plans whose simulations all take the mutual pass:
 // it needs only the grid, so it runs beside the all-pairs pass, and
 // k_mutual_sum adds it to the sums -- the same arithmetic as k_finish.
 template <int LANES>
@@ -1111,12 +1111,10 @@ __global__ __launch_bounds__(64 * kContactWaves) void k_contact_corr(int ntot, c
                                                                      float* __restrict__ corr) {
     const ContactLane c = contact_correction<LANES>(ntot, sims, nsims, starts, sorted, spos, svel, variant);
     if (!c.lead) return;
-    corr[c.i] = c.ex.s;
-    corr[ntot + c.i] = c.ex.c;
-    corr[2 * ntot + c.i] = c.ey.s;
-    corr[3 * ntot + c.i] = c.ey.c;
-    corr[4 * ntot + c.i] = c.ez.s;
-    corr[5 * ntot + c.i] = c.ez.c;
+    float2* rec = reinterpret_cast<float2*>(corr + 6 * (size_t)c.i);
+    rec[0] = make_float2(c.ex.s, c.ex.c);
+    rec[1] = make_float2(c.ey.s, c.ey.c);
+    rec[2] = make_float2(c.ez.s, c.ez.c);
 }
 
 // ---------------------------------------------------------------------------
@@ -1237,12 +1235,9 @@ __global__ __launch_bounds__(64 * kDirectWaves) void k_direct(const float4* __re
                     const float dot = dx * dvx + dy * dvy + dz * dvz;
                     const float fm = contact_pair_mag(law, test, r, r2, dot);
                     const float sf = g * s;
-                    cx.add(fm * dx * invr);
-                    cx.add(-(sf * dx));
-                    cy.add(fm * dy * invr);
-                    cy.add(-(sf * dy));
-                    cz.add(fm * dz * invr);
-                    cz.add(-(sf * dz));
+                    cx.add(fm * dx * invr - sf * dx);
+                    cy.add(fm * dy * invr - sf * dy);
+                    cz.add(fm * dz * invr - sf * dz);
                 }
             } else {
                 const double ddx = (double)pj.x - pi.x, ddy = (double)pj.y - pi.y, ddz = (double)pj.z - pi.z;

@@ -286,6 +286,7 @@ def load() -> C.CDLL:
         "gi_set_batched_tiling": (i, [i, C.c_longlong, i]),
         "gi_set_dispatch_limit": (i, [C.c_longlong]),
         "gi_set_contact_overlap": (i, [C.c_int]),
+        "gi_set_contact_walk": (i, [C.c_int]),
         "gi_serve_run": (i, [vp, i, vp, vp, vp]),
         "gi_spin_body": (i, [vp, vp, i, i, i, C.c_double, C.c_double, C.c_double, vp]),
         "gi_zero_out_drift": (i, [vp, vp, vp]),

@@ -405,6 +405,15 @@ int gi_set_dispatch_limit(long long max_tasks);
  * the other setting is rebuilt on its next use.  No reference counterpart. */
 int gi_set_contact_overlap(int enable);
 
+/* Contact walk (lanes = 0, the default: nine lanes per particle for force
+ * evaluations of up to 49,152 particles, one lane per particle above): the
+ * contact corrections' walk over a particle's 27 neighbour cells, forced to
+ * one lane per particle (1) or one lane per row of three cells (9).  The two
+ * forms share their arithmetic: the corrections are bitwise the same either
+ * way.  Process-wide; a plan built under another setting is rebuilt on its
+ * next use.  GI_E_INVALID_ARG for other values.  No reference counterpart. */
+int gi_set_contact_walk(int lanes);
+
 /* ---- kernel timing (HIP events around the all-pairs force kernel) ------- */
 
 /* Enable/disable per-launch timing (HIP events on the launching stream) of the

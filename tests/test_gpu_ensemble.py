@@ -190,6 +190,34 @@ def test_contact_walk_forms_are_bitwise_the_same(gpu):
         assert np.array_equal(v_all[a:b], v1), s
 
 
+def test_forced_contact_walk_forms_are_bitwise_the_same(gpu):
+    """gi_set_contact_walk forces either walk on any plan: a 60,000-particle
+    impact (one lane by its size) and a 12-simulation batched step give
+    bitwise the same results with nine lanes, one lane and the default."""
+    pos, vel, fc, mc, _ = gpu.make_impact(45000, 15000, seed=41, angle_deg=40.0, speed_vesc=1.1)
+    n = len(pos)
+    print(f"n={n}", flush=True)
+    cfg = gpu.ensemble_configs(12, seed=42, n_range=(3000, 9000))
+    epos, evel, en, efc, emc, _ = gpu.make_ensemble(cfg)
+    out = {}
+    try:
+        for lanes in (0, 1, 9):
+            assert gpu.call("gi_set_contact_walk", lanes) == 0
+            P, V, F = gpu.DeviceArray(n, pos), gpu.DeviceArray(n, vel), gpu.DeviceArray(n, f4(n))
+            gpu.force_collision(P, V, F, n, fc)
+            EP, EV = gpu.DeviceArray(len(epos), epos), gpu.DeviceArray(len(epos), evel)
+            ens = gpu.Ensemble(EP, EV, en, efc, emc)
+            ens.steps(3)
+            gpu.sync()
+            out[lanes] = (F.download(), EP.download(), EV.download())
+            ens.close()
+    finally:
+        gpu.call("gi_set_contact_walk", 0)
+    for lanes in (1, 9):
+        for a, b in zip(out[0], out[lanes]):
+            assert np.array_equal(a, b), lanes
+
+
 def test_resident_pool_table_changes_are_bitwise_packed_ensembles(gpu):
     """An ensemble over a pool of records whose simulation table changes
     (gi_ensemble_set_sims: gaps between simulations, one leaving, one

@@ -142,6 +142,8 @@ def test_invalid_arguments_are_rejected_before_touching_a_device(gi):
     assert lib.gi_set_tiling(2, 4, 0, 0, 0) == -1  # the row-group form needs rows of 256 or 512
     assert lib.gi_set_tiling(0, 0, 0, 0, 0) == 0
     assert lib.gi_set_dispatch_limit(-1) == -1 and lib.gi_set_dispatch_limit(0) == 0
+    assert lib.gi_set_contact_walk(4) == -1 and lib.gi_set_contact_walk(-1) == -1
+    assert lib.gi_set_contact_walk(9) == 0 and lib.gi_set_contact_walk(0) == 0
     assert lib.gi_impact_steps(vp(1), vp(1), vp(1), 10, C.byref(fc), C.byref(mc), 5, 1, None, None, None) == -1
     assert lib.gi_last_error() == -1
     lib.gi_clear_error()
