@@ -1167,6 +1167,12 @@ struct F64Sum {
     __device__ __forceinline__ void add(double x) { s += x; }
     __device__ __forceinline__ double value() const { return s; }
 };
+struct PlainSum {
+    float s;
+    __device__ __forceinline__ void zero() { s = 0.0f; }
+    __device__ __forceinline__ void add(float x) { s += x; }
+    __device__ __forceinline__ float value() const { return s; }
+};
 
 template <bool POT, int S, int MOVE>
 __global__ __launch_bounds__(64 * kDirectWaves) void k_direct(const float4* __restrict__ pos,
@@ -1181,6 +1187,7 @@ __global__ __launch_bounds__(64 * kDirectWaves) void k_direct(const float4* __re
     constexpr int NS = kDirectWaves * S;     // j-streams per particle
     constexpr int TG = kDirectTile / 4;      // groups per stream per float tile
     using Sum = typename std::conditional<POT, F64Sum, CompSum>::type;
+    using CSum = typename std::conditional<POT, F64Sum, PlainSum>::type;
     using Val = typename std::conditional<POT, double, float>::type;
     __shared__ Val red[3][kDirectWaves][64];
     __shared__ float redc[POT ? 1 : 3][kDirectWaves][64];  // the force's compensations
@@ -1211,7 +1218,10 @@ __global__ __launch_bounds__(64 * kDirectWaves) void k_direct(const float4* __re
         const int test = (int)ife + (int)jfe;
         const float g = gravity_product(law, test);
         Sum ux, uy, uz;  // float tiles of kDirectTile j, folded (compensated; FP64 for the potential)
-        Sum cx, cy, cz;
+        // this stream's contact terms of the segment: a handful (the pairs in
+        // contact are dealt over 16 S streams), so a plain float sum rounds
+        // them to ~3 ulp of their magnitudes; FP64 for the potential
+        CSum cx, cy, cz;
         ux.zero();
         uy.zero();
         uz.zero();
@@ -1313,13 +1323,13 @@ __global__ __launch_bounds__(64 * kDirectWaves) void k_direct(const float4* __re
         if constexpr (!POT) {
             sx.add_product(g, ux.s);
             sx.c += g * ux.c;
-            sx.add_sum(cx);
+            sx.add(cx.s);
             sy.add_product(g, uy.s);
             sy.c += g * uy.c;
-            sy.add_sum(cy);
+            sy.add(cy.s);
             sz.add_product(g, uz.s);
             sz.c += g * uz.c;
-            sz.add_sum(cz);
+            sz.add(cz.s);
         } else {
             sx.add(-(double)g * ux.value() + cx.value());
         }
@@ -1335,26 +1345,41 @@ __global__ __launch_bounds__(64 * kDirectWaves) void k_direct(const float4* __re
         red[0][wv][lane] = sx.value();
     }
     __syncthreads();
-    if (wv == 0 && lane < IB && live) {
-        Sum fx, fy, fz;
-        fx.zero();
-        fy.zero();
-        fz.zero();
+    // The force: wavefronts 0-2 take one component each; lane (q, p) adds the
+    // 16 wavefronts' partials of particle p's q-th stream in wavefront order,
+    // then the S streams' sums are added in order (a 16 + S chain per lane
+    // instead of a 16 S chain on one wavefront).  The potential: one FP64
+    // chain on wavefront 0.
+    __shared__ float part[POT ? 1 : 3][2][64];
+    if constexpr (!POT) {
+        if (wv < 3) {
+            CompSum a;
+            a.zero();
 #pragma unroll
-        for (int w = 0; w < kDirectWaves; ++w)
-#pragma unroll
-            for (int q = 0; q < S; ++q) {
-                fx.add(red[0][w][q * IB + lane]);
-                if constexpr (!POT) {
-                    fx.c += redc[0][w][q * IB + lane];
-                    fy.add(red[1][w][q * IB + lane]);
-                    fy.c += redc[1][w][q * IB + lane];
-                    fz.add(red[2][w][q * IB + lane]);
-                    fz.c += redc[2][w][q * IB + lane];
-                }
+            for (int w = 0; w < kDirectWaves; ++w) {
+                a.add(red[wv][w][lane]);
+                a.c += redc[wv][w][lane];
             }
+            part[wv][0][lane] = a.s;
+            part[wv][1][lane] = a.c;
+        }
+        __syncthreads();
+    }
+    if (wv == 0 && lane < IB && live) {
         if constexpr (!POT) {
-            const float f0 = fx.value(), f1 = fy.value(), f2 = fz.value();
+            float f[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                CompSum t;
+                t.zero();
+#pragma unroll
+                for (int q = 0; q < S; ++q) {
+                    t.add(part[c][0][q * IB + lane]);
+                    t.c += part[c][1][q * IB + lane];
+                }
+                f[c] = t.value();
+            }
+            const float f0 = f[0], f1 = f[1], f2 = f[2];
             force[i].x = f0;
             force[i].y = f1;
             force[i].z = f2;
@@ -1393,6 +1418,12 @@ __global__ __launch_bounds__(64 * kDirectWaves) void k_direct(const float4* __re
                 mv.pout[i] = p;
             }
         } else {
+            F64Sum fx;
+            fx.zero();
+#pragma unroll
+            for (int w = 0; w < kDirectWaves; ++w)
+#pragma unroll
+                for (int q = 0; q < S; ++q) fx.add(red[0][w][q * IB + lane]);
             pe[i] = 0.5 * fx.value();
         }
     }
