@@ -65,6 +65,57 @@ def test_generation_phases_match_the_oracle(gpu, oracle):
     assert np.array_equal(pos[:, 3], p0[:, 3]) and np.array_equal(vel[:, 3], v0[:, 3])
 
 
+def _stats_sections(text):
+    """Body sections of targetImpactorStats: {body: {field: [numbers]}}."""
+    import re
+
+    out, cur = {}, None
+    for line in text.splitlines():
+        m = re.search(r"Stats for Body(\d)", line)
+        if m:
+            cur = out.setdefault(int(m.group(1)), {})
+            continue
+        if cur is None:
+            continue
+        nums = [float(x) for x in re.findall(r"[-+]?\d+\.\d+(?:e[-+]\d+)?", line)]
+        for key in ("Mass =", "Radius =", "center of mass", "The average linear velocity =",
+                    "The angular momentum =", "magitude of the angular"):
+            if key in line:
+                cur[key] = nums
+    return out
+
+
+def test_target_impactor_stats_file_against_the_oracle(gpu, oracle, tmp_path):
+    """targetImpactorStats (generateTargetImpactorBasicFunctions.h:590-686): each
+    body's mass, radius, centre of mass, mean velocity and angular momentum of
+    the written final state, in the file's units, against the oracle's
+    restatements of the reference's helpers."""
+    s = _short_setup(gpu, n=1024)
+    pos, vel, u, _ = gpu.generate_bodies(s, seed=8, out_dir=tmp_path)
+    sec = _stats_sections((tmp_path / "targetImpactorStats").read_text())
+    assert set(sec) == {1, 2}
+    b1, b2 = u.nfe1, u.nfe1 + u.nsi1
+    b3, n = b2 + u.nfe2, b2 + u.nfe2 + u.nsi2
+    ranges = {1: (0, b1, b2, u.nfe1, u.nsi1, u.mass_of_body1), 2: (b2, b3, n, u.nfe2, u.nsi2, u.mass_of_body2)}
+    L_c, V_c = u.unit_length, u.unit_length / u.unit_time
+    M_c = u.unit_mass
+    J_c = u.unit_mass * u.unit_length * u.unit_length / u.unit_time
+    for b, (a0, a1, a2, nfe, nsi, nominal) in ranges.items():
+        com = oracle.body_com(pos, a0, a1, a2, u.mass_fe, u.mass_si, nominal)[:3]
+        v = oracle.body_vel(vel, a0, a1, a2, u.mass_fe, u.mass_si, nominal)[:3]
+        L = oracle.body_angmom(pos, vel, a0, a1, a2, u.mass_fe, u.mass_si, nominal)[:3]
+        r = oracle.body_radius(pos, a0, a1, a2, u.mass_fe, u.mass_si, nominal)
+        got = sec[b]
+        assert got["Mass ="][0] == pytest.approx((u.mass_fe * nfe + u.mass_si * nsi) * M_c, rel=1e-5)
+        assert got["Radius ="][0] == pytest.approx(r * L_c, rel=1e-5, abs=1e-5)
+        assert np.allclose(got["center of mass"][:3], com * L_c, rtol=1e-5, atol=2e-6 * r * L_c)
+        vs = np.abs(v).max() * V_c + 1e-9
+        assert np.allclose(got["The average linear velocity ="][:3], v * V_c, rtol=1e-5, atol=1e-5 * vs + 2e-6)
+        Ls = np.linalg.norm(L) * J_c
+        assert np.allclose(got["The angular momentum ="][:3], L * J_c, rtol=1e-5, atol=1e-5 * Ls)
+        assert got["magitude of the angular"][0] == pytest.approx(Ls, rel=1e-5)
+
+
 def test_impact_pipeline_files_resume_and_recenter(gpu, tmp_path):
     s = _short_setup(gpu, n=1024)
     bodies = tmp_path / "bodies"
