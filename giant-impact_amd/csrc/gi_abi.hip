@@ -2263,6 +2263,12 @@ int gi_ensemble_steps(gi_ensemble* e, int n_steps) {
                    &step, {}, true);
     if (rc) return rc;
     for (int s = 0; s < n_steps; ++s) GI_HIP(hipGraphLaunch(step, br.run));
+    // retire the calls that have finished, so a caller that never waits keeps
+    // a bounded list
+    while (!e->pending.empty() && hipEventQuery(e->pending.front()) == hipSuccess) {
+        e->spare.push_back(e->pending.front());
+        e->pending.erase(e->pending.begin());
+    }
     hipEvent_t ev;
     if (!e->spare.empty()) {
         ev = e->spare.back();

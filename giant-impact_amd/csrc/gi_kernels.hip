@@ -620,7 +620,7 @@ struct SumMove {
 // additions, so a wavefront has that many round trips to memory in flight
 // instead of one; the additions keep their order, so the sums are those of
 // the one-at-a-time loop.  POT: x only.
-constexpr int kSumBatch = 8;
+constexpr int kSumBatch = 4;
 
 template <bool POT, typename At>
 __device__ __forceinline__ void add_entries(int k0, int k1, int lane, At at, CompSum& sx, CompSum& sy,

@@ -190,6 +190,27 @@ def test_contact_walk_forms_are_bitwise_the_same(gpu):
         assert np.array_equal(v_all[a:b], v1), s
 
 
+def test_many_step_calls_without_waiting_equal_one_call(gpu):
+    """Twenty gi_ensemble_steps(1) calls queued without gi_ensemble_wait (each
+    records an event; finished ones are retired as calls arrive) leave the
+    same state, bitwise, as one gi_ensemble_steps(20)."""
+    cfg = gpu.ensemble_configs(6, seed=44, n_range=(1500, 5000))
+    pos, vel, n, fc, mc, _ = gpu.make_ensemble(cfg)
+    out = []
+    for split in (True, False):
+        P, V = gpu.DeviceArray(len(pos), pos), gpu.DeviceArray(len(pos), vel)
+        ens = gpu.Ensemble(P, V, n, fc, mc)
+        if split:
+            for _ in range(20):
+                ens.steps(1)
+        else:
+            ens.steps(20)
+        gpu.sync()
+        out.append((P.download(), V.download()))
+        ens.close()
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+
+
 def test_forced_contact_walk_forms_are_bitwise_the_same(gpu):
     """gi_set_contact_walk forces either walk on any plan: a 60,000-particle
     impact (one lane by its size) and a 12-simulation batched step give
