@@ -952,6 +952,7 @@ __device__ __forceinline__ void contact_row(const SimDesc& sd, int t, int i, int
             const float dpy = pj.y - pi.y;
             const float dpz = pj.z - pi.z;
             const float r2 = dpx * dpx + dpy * dpy + dpz * dpz;
+            if (1.0f <= r2) continue;  // then sqrtf(r2) >= 1 too: the root only for candidates that may touch
             const float r = sqrtf(r2);
             if (1.0f <= r) continue;
             const float invr = 1.0f / r;
@@ -1237,6 +1238,7 @@ __global__ __launch_bounds__(64 * kDirectWaves) void k_direct(const float4* __re
             const float dz = pj.z - pi.z;
             if constexpr (!POT) {
                 const float r2 = dx * dx + dy * dy + dz * dz;
+                if (!(r2 < 1.0f)) return;  // then sqrtf(r2) >= 1 too
                 const float r = sqrtf(r2);
                 if (r < 1.0f && j != i) {
                     const float invr = 1.0f / r;
