@@ -298,6 +298,17 @@ def test_separate_force_equals_each_body_alone(oracle):
     assert np.array_equal(F[:b2, :3], oracle.force_collision_f32(pos1, vel1, fc)[:, :3])
     fc.NFe = b3 - b2
     assert np.array_equal(F[b2:, :3], oracle.force_collision_f32(pos2, vel2, fc)[:, :3])
+    # the FP64 twin masks the same pairs: each body's truth, conditioning
+    # sums and near-threshold flags are those of the body alone
+    F64, cond, near = oracle.force_separate_f64(pos, vel, fs)
+    for sl, p_, v_, nfe in ((slice(0, b2), pos1, vel1, b1), (slice(b2, None), pos2, vel2, b3 - b2)):
+        fc.NFe = nfe
+        f1, c1, n1 = oracle.force_collision_f64(p_, v_, fc)
+        assert np.allclose(F64[sl], f1, rtol=1e-13, atol=1e-13 * np.abs(f1).max())
+        assert np.allclose(cond[sl], c1, rtol=1e-13) and np.array_equal(near[sl], n1)
+    # and it is the float routine's truth: the masked float force within 1e-5 of the conditioning sum
+    err = np.linalg.norm(F[:, :3].astype(np.float64) - F64, axis=1)
+    assert np.max(err[~near] / cond[~near]) <= 1e-5
 
 
 # --------------------------------------------------------------------------- units and host helpers
