@@ -15,12 +15,15 @@ oracle's float restatement, 1,000 steps) and N = 65,536 (10,000 steps).
 
 Printed: one JSON line.  value = pair interactions per second of whole steps
 (N^2 per step, self pairs included, as the reference kernel visits them).
-roofline: the all-pairs kernel (k_mutual, each pair evaluated once for both
-particles, or k_gravity, one-sided, as the tuning table picks), N^2 pair
-interactions x 20 FP32 flops against the 157.3 TFLOP/s FP32 peak, timed with
-HIP events on its stream during the timed steps; pass_ms adds the padding and
-per-particle sum kernels of the mutual pass.  cpu_baseline: the oracle's float restatement of the reference
-kernel timed on this host's cores over a bounded sample of rows.
+roofline: the all-pairs kernel (k_mutual_grp or k_mutual, each pair evaluated
+once for both particles, or k_gravity, one-sided, as the tuning table picks),
+N^2 pair interactions x 20 FP32 flops against the 157.3 TFLOP/s FP32 peak, timed
+with HIP events on its stream during the timed steps; pass_ms adds the padding
+and per-particle sum kernels of the mutual pass; frac_of_mix_issue_ceiling sets
+its N^2 / 2 evaluations per second against the issue ceiling of its own
+instruction mix measured by the calibration probe (profiles).  cpu_baseline:
+the oracle's float restatement of the reference kernel timed on this host's
+cores over a bounded sample of rows.
 """
 from __future__ import annotations
 
@@ -56,6 +59,18 @@ def traffic_from_profiles(kernel_prefix: str):
             parts = line.split()
             if len(parts) == 4 and parts[0] == "hbm_bytes_per_launch" and parts[1] == kernel_prefix:
                 return int(parts[2]), float(parts[3])
+    return None
+
+
+def mix_ceiling_from_profiles(kernel: str):
+    """The measured issue ceiling of the kernel's own instruction mix in
+    evaluated pairs per second (profiles/*.txt line
+    'mix_ceiling_evaluated_pairs_per_s <kernel> <value>'), or None."""
+    for f in sorted((REPO / "profiles").glob("*.txt")):
+        for line in f.read_text().splitlines():
+            parts = line.split()
+            if len(parts) == 3 and parts[0] == "mix_ceiling_evaluated_pairs_per_s" and parts[1] == kernel:
+                return float(parts[2])
     return None
 
 
@@ -605,6 +620,7 @@ def main() -> int:
     achieved = pairs * FLOPS_PER_PAIR / grav_s / 1e12
     tr = traffic_from_profiles(plan["kernel"])
     traffic = int(tr[1]) if tr and tr[0] == n else None
+    ceiling = mix_ceiling_from_profiles(plan["kernel"])
     result = {
         "metric": "pair-int/s",
         "value": value,
@@ -637,6 +653,9 @@ def main() -> int:
             "avg_launch_ms": grav_s * 1e3,
             "pass_ms": pass_s * 1e3,
             "frac_pass": pairs * FLOPS_PER_PAIR / pass_s / 1e12 / PEAK_FP32_TFLOPS,
+            # the mutual pass evaluates each pair once for both particles: N^2 / 2
+            # evaluations against the measured issue ceiling of its instruction mix
+            "frac_of_mix_issue_ceiling": (pairs / 2 / grav_s / ceiling) if ceiling else None,
             "plan": {k: plan[k] for k in ("r", "pair_tasks", "chunk_len")},
         },
         "move_kernel": {
