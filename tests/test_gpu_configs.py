@@ -2,7 +2,7 @@
 
   single impact, N = 262,144 (196,608 + 65,536, 45 degrees, 1.1 v_esc) with
       the tuning table's plan (the mutual pass in row groups, R = 8, 1,024 j
-      per task);
+      per task), and a contact-dominated cloud of the same size;
   the batched ensemble of 64 impacts with N ~ U[4,096, 32,768];
   the request loop over a stream with N log-uniform in [2,048, 65,536].
 
@@ -42,6 +42,41 @@ def test_roofline_impact_n262144_against_the_oracle(gpu, oracle):
     tot = Fg[:, :3].astype(np.float64).sum(0)
     assert np.abs(tot).max() <= 1e-5 * np.abs(Fg[:, :3]).astype(np.float64).sum()
     assert np.array_equal(Fg[:, 3], np.zeros(n, np.float32))  # .w never written (the buffer's zeros)
+
+
+def test_contact_dominated_cloud_n262144_against_the_oracle(gpu, oracle):
+    """N = 262,144 through the same plan, in a contact-dominated state: a
+    random cloud of density ~0.95 per unit volume (about four neighbours
+    within r < 1 per particle) with random velocities, so every row's force
+    is mostly contact terms and each pair in contact goes through the
+    all-pairs placeholder and its exact correction.  Rows of both materials
+    against the FP64 truth and the reference's float routine; third law."""
+    n, nfe = 262144, 78643
+    rng = np.random.default_rng(20261207)
+    side = (n / 0.95) ** (1.0 / 3.0)
+    pos, vel = f4(n), f4(n)
+    pos[:, :3] = rng.uniform(0.0, side, (n, 3))
+    vel[:, :3] = rng.normal(0.0, 2.0, (n, 3))
+    fc = gpu.make_impact(10, 0)[2]
+    fc.NFe = nfe
+    print(f"n={n} side={side:.1f}", flush=True)
+    P, V, F = gpu.DeviceArray(n, pos), gpu.DeviceArray(n, vel), gpu.DeviceArray(n, f4(n))
+    gpu.force_collision(P, V, F, n, fc)
+    gpu.sync()
+    assert gpu.plan_info()["kernel"] == "k_mutual_grp"
+    Fg = F.download()
+    contact_rows = 0
+    for a in (0, nfe - 64, nfe, nfe + (n - nfe) // 2, n - 64):
+        F64, cond, near = oracle.force_collision_f64(pos, vel, fc, i_begin=a, i_end=a + 64)
+        Fr = oracle.force_collision_f32(pos, vel, fc, a, a + 64)
+        sl = slice(a, a + 64)
+        met = force_metrics(Fg[sl], F64[sl], cond[sl], near[sl], Fr[sl])
+        print(a, met, flush=True)
+        assert_force_parity(met)
+        contact_rows += int(np.sum(cond[sl] > 10.0 * np.linalg.norm(F64[sl], axis=1)))
+    assert contact_rows > 0  # rows whose net force is a small difference of large terms
+    tot = Fg[:, :3].astype(np.float64).sum(0)
+    assert np.abs(tot).max() <= 1e-5 * np.abs(Fg[:, :3]).astype(np.float64).sum()
 
 
 def _alone(gi, pos, vel, fc, mc, steps):
