@@ -10,9 +10,15 @@
 // mutual pass (3 sub, r^2 by 3 FMA, rsq, 2 mul, 3 + 3 FMA per pair, R pairs
 // per j) for comparison.  Costs are printed in cycles of an assumed 2.4 GHz
 // clock; the ceiling of a mix, sum(n_op x cost_op), does not depend on that
-// assumption.  Each thread reads in[] and writes out[tid] only.
+// assumption.  A last section times where a v_rsq_f32's consumer sits: the
+// same instructions (12 independent FMAs, one v_rsq_f32 and a multiply of its
+// result) with the multiply right after the v_rsq_f32 (after its one wait
+// state), three instructions later, or twelve later (inline asm of register
+// operations only, so the order is the one written).  Each thread reads in[]
+// and writes out[tid] only.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 
@@ -93,10 +99,49 @@ __global__ __launch_bounds__(256) void k_pairmix(float* __restrict__ out, const 
     out[tid] = t;
 }
 
+// 12 independent FMAs per iteration on a0..a11 (asm operands 0..11); t (12)
+// takes v_rsq_f32 of u (13), m (14) the product t * t; x (15), y (16).
+#define GI_FMA(k) "v_fma_f32 %" #k ", %" #k ", %15, %16\n\t"
+#define GI_FMA3(a, b, c) GI_FMA(a) GI_FMA(b) GI_FMA(c)
+#define GI_FMA12 GI_FMA3(0, 1, 2) GI_FMA3(3, 4, 5) GI_FMA3(6, 7, 8) GI_FMA3(9, 10, 11)
+#define GI_RSQ "v_rsq_f32_e64 %12, %13\n\t"
+#define GI_MUL "v_mul_f32_e32 %14, %12, %12\n\t"
+
+template <int P>
+__global__ __launch_bounds__(256) void k_rsq_place(float* __restrict__ out, const float* __restrict__ in, int iters) {
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const float x = in[tid & 1023], y = in[(tid + 5) & 1023];
+    float a0 = x, a1 = x + 1.0f, a2 = x + 2.0f, a3 = x + 3.0f, a4 = x + 4.0f, a5 = x + 5.0f, a6 = x + 6.0f,
+          a7 = x + 7.0f, a8 = x + 8.0f, a9 = x + 9.0f, a10 = x + 10.0f, a11 = x + 11.0f;
+    float t = 0.0f, u = y + 1.0f, m = 0.0f;
+    for (int it = 0; it < iters; ++it) {
+#define GI_OPS                                                                                                   \
+    : "+v"(a0), "+v"(a1), "+v"(a2), "+v"(a3), "+v"(a4), "+v"(a5), "+v"(a6), "+v"(a7), "+v"(a8), "+v"(a9), "+v"(a10), \
+      "+v"(a11), "+v"(t), "+v"(u), "+v"(m)                                                                         \
+    : "v"(x), "v"(y)
+        if constexpr (P == 0)
+            asm volatile(GI_FMA12 GI_OPS);
+        else if constexpr (P == 1)
+            asm volatile(GI_FMA12 GI_MUL GI_OPS);
+        else if constexpr (P == 2)
+            asm volatile(GI_RSQ "s_nop 0\n\t" GI_MUL GI_FMA12 GI_OPS);
+        else if constexpr (P == 3)
+            asm volatile(GI_RSQ GI_FMA3(0, 1, 2) GI_MUL GI_FMA3(3, 4, 5) GI_FMA3(6, 7, 8) GI_FMA3(9, 10, 11) GI_OPS);
+        else
+            asm volatile(GI_RSQ GI_FMA12 GI_MUL GI_OPS);
+#undef GI_OPS
+    }
+    out[tid] = a0 + a1 + a2 + a3 + a4 + a5 + a6 + a7 + a8 + a9 + a10 + a11 + t + u + m;
+}
+
 using Launch = void (*)(float*, const float*, int, int);
 template <int OP>
 static void launch_op(float* d, const float* in, int grid, int iters) {
     hipLaunchKernelGGL((k_op<OP>), dim3(grid), dim3(256), 0, 0, d, in, iters);
+}
+template <int P>
+static void launch_place(float* d, const float* in, int grid, int iters) {
+    hipLaunchKernelGGL((k_rsq_place<P>), dim3(grid), dim3(256), 0, 0, d, in, iters);
 }
 template <int R>
 static void launch_pairs(float* d, const float* in, int grid, int iters) {
@@ -150,6 +195,29 @@ int main() {
             const double ms = time_ms(R == 4 ? launch_pairs<4> : launch_pairs<8>, d, in, grid, iters);
             const double pairs = (double)grid * 256 * iters * R;
             std::printf("pairmix R=%d waves/SIMD %d: %8.3f ms  %.4e evaluated pairs/s\n", R, wps, ms, pairs / (ms * 1e-3));
+        }
+    }
+    // v_rsq_f32 placement: a warm-up of ~0.3 s first, then the best of five
+    // runs of each pattern; cost of the v_rsq_f32 and its multiply in each
+    // placement = its time minus that of the 12 FMAs alone.
+    const char* pnames[5] = {"12 fma", "12 fma + mul", "rsq, wait, mul, 12 fma", "rsq, 3 fma, mul, 9 fma",
+                             "rsq, 12 fma, mul"};
+    const Launch places[5] = {launch_place<0>, launch_place<1>, launch_place<2>, launch_place<3>, launch_place<4>};
+    const int piters = 40000;
+    for (int k = 0; k < 40; ++k) places[0](d, in, cus * 8, piters);
+    CHECK(hipGetLastError());
+    CHECK(hipDeviceSynchronize());
+    for (int wps : {5, 6, 8}) {
+        const int grid = cus * wps;
+        double base = 0.0;
+        for (int p = 0; p < 5; ++p) {
+            double best = 1e30;
+            for (int rep = 0; rep < 5; ++rep) best = std::min(best, time_ms(places[p], d, in, grid, piters));
+            // cycles per iteration of one wavefront, with wps of them sharing a SIMD
+            const double cyc = best * 1e-3 * clk / ((double)wps * piters);
+            if (p == 0) base = cyc;
+            std::printf("placement waves/SIMD %d  %-24s %7.3f ms  %6.2f cycles per iteration  (+%5.2f)\n", wps,
+                        pnames[p], best, cyc, cyc - base);
         }
     }
     CHECK(hipFree(d));
