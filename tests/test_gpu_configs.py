@@ -1,7 +1,7 @@
 """The benchmark workloads at their real sizes, against the oracle:
 
   single impact, N = 262,144 (196,608 + 65,536, 45 degrees, 1.1 v_esc) with
-      the tuning table's plan (the mutual pass in row groups, R = 8, 1,024 j
+      the tuning table's plan (the mutual pass in row groups, R = 8, 1,536 j
       per task), and a contact-dominated cloud of the same size;
   the batched ensemble of 64 impacts with N ~ U[4,096, 32,768];
   the request loop over a stream with N log-uniform in [2,048, 65,536].
@@ -27,7 +27,7 @@ def test_roofline_impact_n262144_against_the_oracle(gpu, oracle):
     gpu.force_collision(P, V, F, n, fc)
     gpu.sync()
     plan = gpu.plan_info()
-    assert (plan["kernel"], plan["r"], plan["chunk_len"]) == ("k_mutual_grp", 8, 1024), plan
+    assert (plan["kernel"], plan["r"], plan["chunk_len"]) == ("k_mutual_grp", 8, 1536), plan
     Fg = F.download()
     nfe = fc.NFe
     # 5 x 64 rows: target iron, impactor iron (the last irons), first and
