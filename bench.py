@@ -140,7 +140,7 @@ def ensemble_leg(gi, n_sims: int, steps: int, seq_steps: int = 100, check_every:
     # alone are timed too
     t_batched = t_steps_only = 0.0
     done = 0
-    dE_series, dL_series = [], []
+    dE_series, dL_series, seg_ms = [], [], []
     scaleL = np.maximum(np.linalg.norm(c0[:, 6:9], axis=1), 1e-30)
     while done < steps:
         k = min(check_every, steps - done)
@@ -151,6 +151,7 @@ def ensemble_leg(gi, n_sims: int, steps: int, seq_steps: int = 100, check_every:
         c = ens.conserved()
         t_batched += time.perf_counter() - t0
         t_steps_only += t1 - t0
+        seg_ms.append(1e3 * (t1 - t0) / k)
         done += k
         dE_series.append(float((np.abs(c[:, 2] - c0[:, 2]) / np.abs(c0[:, 2])).max()))
         dL_series.append(float((np.linalg.norm(c[:, 6:9] - c0[:, 6:9], axis=1) / scaleL).max()))
@@ -210,6 +211,7 @@ def ensemble_leg(gi, n_sims: int, steps: int, seq_steps: int = 100, check_every:
         "fp64_angmom_rel_change_max": float(dL.max()),
         "fp64_energy_rel_change_series": dE_series,
         "fp64_angmom_rel_change_series": dL_series,
+        "segment_ms_per_step": seg_ms,
     }
 
 

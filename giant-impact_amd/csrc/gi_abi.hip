@@ -773,7 +773,7 @@ static int launch_grid(Workspace* ws, const float4* pos, const float4* vel, hipS
     const int nb = nblk(m, 1024);
     hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(256), 0, st, ws->counts.p, m, ws->starts.p,
                        ws->block_sums.p);
-    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, st, ws->block_sums.p, nb, ws->starts.p + m);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanTopThreads), 0, st, ws->block_sums.p, nb, ws->starts.p + m);
     hipLaunchKernelGGL(k_scan_add, dim3(nblk(m, 256)), dim3(256), 0, st, ws->starts.p, m, ws->block_sums.p);
     hipLaunchKernelGGL(k_grid_scatter, dim3(nblk(p.ntot, 256)), dim3(256), 0, st, p.ntot, ws->cellhash.p,
                        ws->starts.p, ws->counts.p, ws->sorted.p);
@@ -1760,7 +1760,7 @@ int gi_find_earth_moon(const gi_float4* pos, const gi_float4* vel, int n, int nf
     hipLaunchKernelGGL(k_cc_count, dim3(nblk(n, 256)), dim3(256), 0, st, P, n, inv_c, m - 1, w.counts.p,
                        w.cellhash.p);
     hipLaunchKernelGGL(k_scan_blocks, dim3(nbs), dim3(256), 0, st, w.counts.p, (int)m, w.starts.p, w.block_sums.p);
-    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, st, w.block_sums.p, nbs, w.starts.p + m);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanTopThreads), 0, st, w.block_sums.p, nbs, w.starts.p + m);
     hipLaunchKernelGGL(k_scan_add, dim3(nblk(m, 256)), dim3(256), 0, st, w.starts.p, (int)m, w.block_sums.p);
     hipLaunchKernelGGL(k_grid_scatter, dim3(nblk(n, 256)), dim3(256), 0, st, n, w.cellhash.p, w.starts.p,
                        w.counts.p, w.sorted.p);

@@ -829,33 +829,39 @@ __global__ __launch_bounds__(256) void k_scan_blocks(const int* __restrict__ in,
     if (threadIdx.x == 255) block_sums[blockIdx.x] = sh[255];
 }
 
-// Exclusive scan of the block sums in one block (nb <= 1024 * 4).
-__global__ __launch_bounds__(1024) void k_scan_top(int* __restrict__ block_sums, int nb,
-                                                   int* __restrict__ total_out) {
-    __shared__ int sh[1024];
-    const int base = threadIdx.x * 4;
-    int v[4];
+// Exclusive scan of the block sums in one workgroup of kScanTopThreads
+// threads, 16 sums each (nb <= 4096).  The workgroup is kept to one
+// wavefront per SIMD: it runs on a side stream beside the pair kernel, and a
+// 1,024-thread workgroup (four wavefronts on every SIMD of one CU) waited
+// for a CU to drain for most of the pair kernel's run.
+constexpr int kScanTopThreads = 256;
+__global__ __launch_bounds__(kScanTopThreads) void k_scan_top(int* __restrict__ block_sums, int nb,
+                                                              int* __restrict__ total_out) {
+    constexpr int T = kScanTopThreads, E = 4096 / kScanTopThreads;
+    __shared__ int sh[T];
+    const int base = threadIdx.x * E;
+    int v[E];
     int local = 0;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < E; ++k) {
         v[k] = (base + k < nb) ? block_sums[base + k] : 0;
         local += v[k];
     }
     sh[threadIdx.x] = local;
     __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        int add = threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
+    for (int off = 1; off < T; off <<= 1) {
+        int add = (int)threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
         __syncthreads();
         sh[threadIdx.x] += add;
         __syncthreads();
     }
     int run = sh[threadIdx.x] - local;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < E; ++k) {
         if (base + k < nb) block_sums[base + k] = run;
         run += v[k];
     }
-    if (threadIdx.x == 1023) *total_out = sh[1023];
+    if (threadIdx.x == T - 1) *total_out = sh[T - 1];
 }
 
 __global__ void k_scan_add(int* __restrict__ out, int m, const int* __restrict__ block_sums) {
