@@ -83,7 +83,7 @@ def tune_batched(quick: bool):
     best = None
     Ls = [1024, 2048, 4096, 8192, 16384] if not quick else [4096]
     cands = [(R, L, 0) for R, L in itertools.product([1, 2], Ls)]
-    cands += [(R, L, v) for R, L, v in itertools.product([4, 8], [256, 384, 512, 768, 1024, 2048], [2, 4])]
+    cands += [(R, L, v) for R, L, v in itertools.product([4, 8], [256, 384, 512, 768, 1024, 1536, 2048], [2, 4])]
     for R, L, v in cands:
         call("gi_set_batched_tiling", R, L, v)
         P.upload(pos)
@@ -126,7 +126,7 @@ def main(argv=None) -> int:
     # at N = 262,144); --variants sweeps it again
     variants = [0, 1] if args.variants else [0]
     targets = [4096, 8192, 16384, 32768, 65536, 131072] if not args.quick else [32768]
-    mutual_Ls = [192, 256, 384, 512, 1024, 2048] if not args.quick else [512]
+    mutual_Ls = [192, 256, 384, 512, 768, 1024, 1536, 2048] if not args.quick else [512]
     rows = []
     only = {int(x) for x in args.classes.split(",") if x.strip()}
     kept = {}
