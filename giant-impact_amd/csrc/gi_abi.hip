@@ -431,9 +431,10 @@ static int plan_mutual(std::vector<SimDesc>& sims, const Tiling& tl, bool batche
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return swork[a] > swork[b]; });
     p.sblocks.resize(order.size());
     for (size_t k = 0; k < order.size(); ++k) p.sblocks[k] = sblocks[order[k]];
-    // Longest first (VALU ops per lane: R*12 per j one-sided, R*15+6 mutual).
+    // Longest first (VALU ops per lane and j: R*15+6; a diagonal task
+    // evaluates about half its row against itself).
     auto cost = [&](const MutualTask& t) {
-        return (long long)(t.j1 - t.j0) * (t.diag ? 12 * tl.R : 15 * tl.R + 6);
+        return (long long)(t.j1 - t.j0) * (t.diag ? (15 * tl.R) / 2 + 6 : 15 * tl.R + 6);
     };
     std::stable_sort(tasks.begin(), tasks.end(),
                      [&](const MutualTask& a, const MutualTask& b) { return cost(a) > cost(b); });

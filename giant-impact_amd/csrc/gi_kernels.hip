@@ -162,15 +162,17 @@ __device__ __forceinline__ int find_sim(const SimDesc* __restrict__ sims, int ns
 // Simulations are copied into a padded layout (iron then silicate, each
 // rounded up to whole rows with ghosts far away) so that every row and every
 // 64-block is of one material and the gravity product is one factor per task.
-// The row against itself ("diag" task) runs one-sided with wave-uniform
-// scalar loads.  Outputs: per task an i-side slot (3 planes of 64*R floats)
+// The row against itself ("diag" task) takes each of its pairs once too (a
+// one-sided form at R = 8 made the 64-simulation batched step 1.5% slower).
+// Outputs: per task an i-side slot (3 planes of 64*R floats)
 // and per (row, j-block) a j-side slab entry; k_mutual_sum adds them in
 // compensated float sums in a fixed order.  Both sides accumulate in plain
 // float: a j-side entry sums
 // 64*R terms, an i-side slot at most one chunk (<= 4096 j, the planner's
 // cap), so the rounding error stays at a few 1e-6 of the sum of the terms'
-// magnitudes; leaving out tile sums and compensation keeps R = 8 at 94 VGPRs
-// (five wavefronts per SIMD instead of four, 2% faster).
+// magnitudes; leaving out tile sums and compensation keeps the register
+// count low enough for five (k_mutual) or six (k_mutual_grp) wavefronts per
+// SIMD at R = 8.
 
 __device__ __forceinline__ float rotate_lanes(float v) {
     // wave_ror:1 -- lane l reads lane l-1 (mod 64).
@@ -222,30 +224,7 @@ __device__ __forceinline__ void mutual_wave_task(const MutualTask& t, const SimD
     // j_((127 - m) mod 64), so step k reads q[63 - lane + k] =
     // j_((lane - k) mod 64) -- the j whose accumulator lane `lane` holds
     // after k rotations -- at the immediate offset 16 k.
-    if (R > 4 && t.diag) {
-        // The row against itself, one-sided (includes the self pair): at
-        // R = 8 this keeps the off-diagonal loop's register allocation (the
-        // mutual form below costs that loop 4%).
-#pragma unroll 4
-        for (int j = t.j0; j < t.j1; ++j) {
-            const float4 pj = P[j];
-#pragma unroll
-            for (int r = 0; r < R; ++r) {
-                const float dx = pj.x - xi[r];
-                const float dy = pj.y - yi[r];
-                const float dz = pj.z - zi[r];
-                if constexpr (!POT) {
-                    const float s = fast_inv_r3(fast_r2(dx, dy, dz));
-                    ax[r] = __builtin_fmaf(s, dx, ax[r]);
-                    ay[r] = __builtin_fmaf(s, dy, ay[r]);
-                    az[r] = __builtin_fmaf(s, dz, az[r]);
-                } else {
-                    const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
-                    ax[r] += (r2 > 0.0f && r2 < kGhostR2) ? __builtin_amdgcn_rsqf(r2) : 0.0f;
-                }
-            }
-        }
-    } else if (t.diag) {
+    if (t.diag) {
         // The row against itself, each pair once: its 64-blocks m holding a
         // real particle against the row's sub-blocks r < m (both sides, as
         // below), and against sub-block m itself over rotations 1..32 (lanes
