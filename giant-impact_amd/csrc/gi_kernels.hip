@@ -512,9 +512,9 @@ __device__ __forceinline__ void mutual_group_task(const GroupTask& gt, const Sim
     }
 }
 
-// Six wavefronts per SIMD (at most 80 VGPRs; R = 8 needs 84 unconstrained,
-// the compiler spills three values outside the inner loop): 2.5% faster at
-// N = 262,144 than five.
+// Six wavefronts per SIMD (at most 80 VGPRs; R = 8 needs a few more
+// unconstrained, and the compiler spills two or three values outside the
+// inner loop): 2.5% faster at N = 262,144 than five.
 template <int R, bool POT>
 __global__ __launch_bounds__(64 * kMutualWaves) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_mutual_grp(
     const float4* __restrict__ gpos, const GroupTask* __restrict__ gtasks, int ntasks,
