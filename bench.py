@@ -385,7 +385,8 @@ def drift_leg(gi, steps_small: int, steps_large: int, check_every: int, steps_he
                     "oracle_1thread_seconds": t_ref1, "oracle_1thread_pair_per_s": pairs / t_ref1,
                     "oracle_threads_agree": bool(np.array_equal(po, po1) and np.array_equal(vo, vo1))}
     # The same comparison over 16 seeds: at this size both changes are a few float roundings of L,
-    # so one seed says little about which path drifts more; the criterion is the median.
+    # so one seed says little about which path drifts more; the criterion is the median, of the
+    # two paths' values and of the per-seed ratios (each seed is one initial state run both ways).
     seeds = [20261400 + k for k in range(16)]
     gl, rl, ge, re_, gp, rp = [], [], [], [], [], []
     for sd in seeds:
@@ -414,7 +415,13 @@ def drift_leg(gi, steps_small: int, steps_large: int, check_every: int, steps_he
                           "reference_float_momentum_median": float(np.median(rp)),
                           "median_no_worse_than_reference": bool(np.median(gl) <= np.median(rl) and
                                                                  np.median(gp) <= np.median(rp)),
-                          "seeds_gpu_angmom_le_reference": int(sum(a <= b for a, b in zip(gl, rl)))}
+                          "seeds_gpu_angmom_le_reference": int(sum(a <= b for a, b in zip(gl, rl))),
+                          # paired by seed (the same initial state on both paths): the median of the
+                          # per-seed ratios GPU / reference
+                          "paired_median_ratio_angmom": float(np.median(np.array(gl) / np.array(rl))),
+                          "paired_median_ratio_momentum": float(np.median(np.array(gp) / np.array(rp))),
+                          "paired_no_worse_than_reference": bool(np.median(np.array(gl) / np.array(rl)) <= 1.0 and
+                                                                 np.median(np.array(gp) / np.array(rp)) <= 1.0)}
     # A simulation of the 64-impact ensemble (N = 13,255), 200 batched steps
     # against 200 steps of the reference float path.
     cfg = gi.ensemble_configs(3, seed=20261205)[1]
