@@ -680,11 +680,11 @@ static int check_plan_indexes(const Plan& p) {
                 const long long len = sd.gn - (g + 1) * WB;
                 const long long o = q - (g + 1) * WB;
                 GI_CHECK(o >= 0 && o < len, "sum: group entry");
-                GI_CHECK(sd.jbase + 3 * jside_row_offset(g, sd.gn, WB) + 2 * len + o < p.jside_len, "sum: J");
+                GI_CHECK(sd.jbase + 3 * (jside_row_offset(g, sd.gn, WB) + o) + 2 < p.jside_len, "sum: J");
             }
             for (int r2 = group * W; r2 < row; ++r2) {
                 const long long o = q - (long long)(r2 + 1) * B;
-                GI_CHECK(o >= 0 && o < ilen && sd.jibase + 3 * ilen * r2 + 2 * ilen + o < p.jintra_len,
+                GI_CHECK(o >= 0 && o < ilen && sd.jibase + 3 * (ilen * r2 + o) + 2 < p.jintra_len,
                          "sum: in-group entry");
             }
         }

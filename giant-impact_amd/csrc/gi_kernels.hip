@@ -164,8 +164,10 @@ __device__ __forceinline__ int find_sim(const SimDesc* __restrict__ sims, int ns
 // 64-block is of one material and the gravity product is one factor per task.
 // The row against itself ("diag" task) takes each of its pairs once too (a
 // one-sided form at R = 8 made the 64-simulation batched step 1.5% slower).
-// Outputs: per task an i-side slot (3 planes of 64*R floats)
-// and per (row, j-block) a j-side slab entry; k_mutual_sum adds them in
+// Outputs: per task an i-side slot (an (x, y, z) record per place of the
+// row) and per (row, j-block) a j-side slab entry (an (x, y, z) record per
+// j), so that k_mutual_sum reads 768 contiguous bytes per wavefront and
+// entry; k_mutual_sum adds them in
 // compensated float sums in a fixed order.  Both sides accumulate in plain
 // float: a j-side entry sums
 // 64*R terms, an i-side slot at most one chunk (<= 4096 j, the planner's
@@ -197,13 +199,13 @@ constexpr int kMutualWaves = 4;  // wavefronts per block of k_mutual
 
 // One wavefront's task t of simulation sd (P: its padded positions, q: the
 // wavefront's 128 float4 of LDS): a row against itself (t.diag) or against
-// j beyond the row, whose j-side sums go to J (three planes of len entries,
-// entry j - (t.row + 1) * B); the row's i-side sums go to slot t.slot.
+// j beyond the row, whose j-side sums go to J (one (x, y, z) record per
+// entry, entry j - (t.row + 1) * B); the row's i-side sums go to slot t.slot
+// (one (x, y, z) record per place of the row).  POT: x only.
 template <int R, bool POT>
 __device__ __forceinline__ void mutual_wave_task(const MutualTask& t, const SimDesc& sd,
                                                  const float4* __restrict__ P, float4* __restrict__ q,
-                                                 float* __restrict__ iside, float* __restrict__ J, long long len,
-                                                 int lane) {
+                                                 float* __restrict__ iside, float* __restrict__ J, int lane) {
     constexpr int B = 64 * R;
     const int ib = t.row * B;
     const bool ife = ib < sd.gfe;
@@ -339,11 +341,11 @@ __device__ __forceinline__ void mutual_wave_task(const MutualTask& t, const SimD
             // the potential is symmetric.
             const long long o = (long long)(J0 - first) + lane;
             if constexpr (!POT) {
-                J[o] = -fx * g;
-                J[len + o] = -fy * g;
-                J[2 * len + o] = -fz * g;
+                J[3 * o] = -fx * g;
+                J[3 * o + 1] = -fy * g;
+                J[3 * o + 2] = -fz * g;
             } else {
-                J[o] = fx * g;
+                J[3 * o] = fx * g;
             }
         }
     }
@@ -351,10 +353,10 @@ __device__ __forceinline__ void mutual_wave_task(const MutualTask& t, const SimD
     float* __restrict__ I = iside + (size_t)t.slot * (3 * B);
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-        I[r * 64 + lane] = ax[r] * g;
+        I[3 * (r * 64 + lane)] = ax[r] * g;
         if constexpr (!POT) {
-            I[B + r * 64 + lane] = ay[r] * g;
-            I[2 * B + r * 64 + lane] = az[r] * g;
+            I[3 * (r * 64 + lane) + 1] = ay[r] * g;
+            I[3 * (r * 64 + lane) + 2] = az[r] * g;
         }
     }
 }
@@ -374,7 +376,7 @@ __global__ __launch_bounds__(64 * kMutualWaves) void k_mutual(const float4* __re
     const SimDesc& sd = sims[t.sim];
     __shared__ float4 lq[kMutualWaves][128];
     float* __restrict__ J = jside + sd.jbase + 3 * jside_row_offset(t.row, sd.gn, B);
-    mutual_wave_task<R, POT>(t, sd, gpos + sd.gbase, lq[wv], iside, J, sd.gn - (long long)(t.row + 1) * B, lane);
+    mutual_wave_task<R, POT>(t, sd, gpos + sd.gbase, lq[wv], iside, J, lane);
 }
 
 // ---------------------------------------------------------------------------
@@ -420,7 +422,6 @@ __device__ __forceinline__ void mutual_group_task(const GroupTask& gt, const Sim
         ax[r] = ay[r] = az[r] = 0.0f;
     }
     const int gend = min((gt.group + 1) * W, nrows) * B;  // first j of the group's slab
-    const long long len = sd.gn - gend;
     float* __restrict__ J = jside + sd.jbase + 3 * jside_row_offset(gt.group, sd.gn, W * B);
     // Each wavefront stages a quarter of every block: lanes 0..15 of wave wv
     // own j = 16 wv + lane, stored twice as k_mutual lays a block out.
@@ -495,7 +496,7 @@ __device__ __forceinline__ void mutual_group_task(const GroupTask& gt, const Sim
             float a = red[buf][wv][0][lane];
 #pragma unroll
             for (int w = 1; w < W; ++w) a += red[buf][wv][w][lane];
-            J[wv * len + (J0 - gend) + lane] = a;
+            J[3 * ((J0 - gend) + lane) + wv] = a;
         }
         buf ^= 1;
     }
@@ -503,10 +504,10 @@ __device__ __forceinline__ void mutual_group_task(const GroupTask& gt, const Sim
         float* __restrict__ I = iside + (size_t)(rows[sd.row0 + row].slot0 + gt.piece) * (3 * B);
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            I[r * 64 + lane] = ax[r] * g;
+            I[3 * (r * 64 + lane)] = ax[r] * g;
             if constexpr (!POT) {
-                I[B + r * 64 + lane] = ay[r] * g;
-                I[2 * B + r * 64 + lane] = az[r] * g;
+                I[3 * (r * 64 + lane) + 1] = ay[r] * g;
+                I[3 * (r * 64 + lane) + 2] = az[r] * g;
             }
         }
     }
@@ -532,7 +533,7 @@ __global__ __launch_bounds__(64 * kMutualWaves) __attribute__((amdgpu_waves_per_
         if (t.j1 <= t.j0) return;  // a filler of the last workgroup
         const SimDesc& sd = sims[t.sim];
         float* __restrict__ J = jintra + sd.jibase + 3LL * (kGroupRows - 1) * B * t.row;
-        mutual_wave_task<R, POT>(t, sd, gpos + sd.gbase, lq[wv], iside, J, (long long)(kGroupRows - 1) * B, lane);
+        mutual_wave_task<R, POT>(t, sd, gpos + sd.gbase, lq[wv], iside, J, lane);
         return;
     }
     const SimDesc& sd = sims[gt.sim];
@@ -595,12 +596,12 @@ struct SumMove {
 };
 
 // Entries k0 .. k1 - 1 of one list (slots, or slab entries) added to
-// (sx, sy, sz) in order.  at(k, p, plane) gives the wavefront-uniform address
-// of entry k's x value for lane 0 (lane l reads p[l]) and the distance to its
-// y and z values.  The loads of kSumBatch entries are issued before their
-// additions, so a wavefront has that many round trips to memory in flight
-// instead of one; the additions keep their order, so the sums are those of
-// the one-at-a-time loop.  POT: x only.
+// (sx, sy, sz) in order.  at(k) gives the wavefront-uniform address of entry
+// k's record for lane 0: lane l reads the (x, y, z) record at p + 3 l, so a
+// wavefront reads 768 contiguous bytes per entry.  The loads of kSumBatch
+// entries are issued before their additions, so a wavefront has that many
+// round trips to memory in flight instead of one; the additions keep their
+// order, so the sums are those of the one-at-a-time loop.  POT: x only.
 constexpr int kSumBatch = 4;
 
 template <bool POT, typename At>
@@ -611,13 +612,11 @@ __device__ __forceinline__ void add_entries(int k0, int k1, int lane, At at, Com
         float vx[kSumBatch], vy[kSumBatch], vz[kSumBatch];
 #pragma unroll
         for (int u = 0; u < kSumBatch; ++u) {
-            const float* p;
-            long long plane;
-            at(k + u, p, plane);
-            vx[u] = p[lane];
+            const float* p = at(k + u) + 3 * lane;
+            vx[u] = p[0];
             if constexpr (!POT) {
-                vy[u] = p[plane + lane];
-                vz[u] = p[2 * plane + lane];
+                vy[u] = p[1];
+                vz[u] = p[2];
             }
         }
 #pragma unroll
@@ -630,13 +629,11 @@ __device__ __forceinline__ void add_entries(int k0, int k1, int lane, At at, Com
         }
     }
     for (; k < k1; ++k) {
-        const float* p;
-        long long plane;
-        at(k, p, plane);
-        sx.add(p[lane]);
+        const float* p = at(k) + 3 * lane;
+        sx.add(p[0]);
         if constexpr (!POT) {
-            sy.add(p[plane + lane]);
-            sz.add(p[2 * plane + lane]);
+            sy.add(p[1]);
+            sz.add(p[2]);
         }
     }
 }
@@ -676,18 +673,14 @@ __global__ __launch_bounds__(64 * kSumSplit) void k_mutual_sum(const SumBlock* _
         const RowDesc rd = rows[sd.row0 + row];
         add_entries<POT>(
             rd.nslot * w / kSumSplit, rd.nslot * (w + 1) / kSumSplit, lane,
-            [&](int k, const float*& p, long long& plane) {
-                p = iside + (size_t)(rd.slot0 + k) * (3 * B) + o0;
-                plane = B;
-            },
+            [&](int k) { return iside + (size_t)(rd.slot0 + k) * (3 * B) + 3 * o0; },
             sx, sy, sz);
         // the j-side entries of the earlier groups of W rows (W = 1: rows) ...
         const int group = row / W, WB = W * B;
         add_entries<POT>(
             group * w / kSumSplit, group * (w + 1) / kSumSplit, lane,
-            [&](int r, const float*& p, long long& plane) {
-                plane = sd.gn - (long long)(r + 1) * WB;
-                p = jside + sd.jbase + 3 * jside_row_offset(r, sd.gn, WB) + (blk.q0 - (long long)(r + 1) * WB);
+            [&](int r) {
+                return jside + sd.jbase + 3 * (jside_row_offset(r, sd.gn, WB) + (blk.q0 - (long long)(r + 1) * WB));
             },
             sx, sy, sz);
         // ... and of the earlier rows of its own group
@@ -695,10 +688,7 @@ __global__ __launch_bounds__(64 * kSumSplit) void k_mutual_sum(const SumBlock* _
         const long long ilen = (long long)(W - 1) * B;
         add_entries<POT>(
             group * W + in_group * w / kSumSplit, group * W + in_group * (w + 1) / kSumSplit, lane,
-            [&](int r, const float*& p, long long& plane) {
-                p = jintra + sd.jibase + 3 * ilen * r + (blk.q0 - (long long)(r + 1) * B);
-                plane = ilen;
-            },
+            [&](int r) { return jintra + sd.jibase + 3 * (ilen * r + (blk.q0 - (long long)(r + 1) * B)); },
             sx, sy, sz);
     }
     red[0][0][w][lane] = sx.s;
