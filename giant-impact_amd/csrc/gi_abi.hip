@@ -321,8 +321,22 @@ static int plan_mutual_groups(const std::vector<SimDesc>& sims, const Tiling& tl
                 slot += (int)pieces.size();
                 const int rb = r * B;
                 const bool rowfe = rb < s.gfe;
-                tasks.push_back(MutualTask{(int)si, r, rb, std::min(rb + B, rowfe ? fe_real : si_real), slot++,
-                                           rowfe ? 1 : 0, 1, 0});
+                // the row against itself: block m costs m + 1 sub-block
+                // passes; cut in pieces of about the work of a piece of L
+                const int re = std::min(rb + B, rowfe ? fe_real : si_real);
+                const long long nb = (re - rb) / 64, total = nb * (nb + 1) / 2;
+                const long long target = std::max(1LL, L / 64 * tl.R);
+                const long long nd = std::min(nb, (total + target - 1) / target);
+                long long m = 0, done = 0;  // blocks [0, m) taken, their work
+                for (long long k = 1; k <= nd && m < nb; ++k) {
+                    const long long m_begin = m, goal = total * k / nd;
+                    do done += ++m;  // block m - 1, work m
+                    while (m < nb && done + (m + 1) <= goal);
+                    if (k == nd)
+                        while (m < nb) done += ++m;
+                    tasks.push_back(MutualTask{(int)si, r, rb + (int)(64 * m_begin), rb + (int)(64 * m), slot++,
+                                               rowfe ? 1 : 0, 1, 0});
+                }
                 // the rest of the group: iron part, then silicate part
                 const int a = rb + B, b = gend;
                 auto push_pieces = [&](int a0, int a1, int jfe) {
@@ -451,10 +465,12 @@ static int plan_mutual(std::vector<SimDesc>& sims, const Tiling& tl, bool batche
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return swork[a] > swork[b]; });
     p.sblocks.resize(order.size());
     for (size_t k = 0; k < order.size(); ++k) p.sblocks[k] = sblocks[order[k]];
-    // Longest first (VALU ops per lane and j: R*15+6; a diagonal task
-    // evaluates about half its row against itself).
+    // Longest first (VALU ops per lane and j: R*15+6; a diagonal task's
+    // block m passes over m + 1 of the row's sub-blocks).
     auto cost = [&](const MutualTask& t) {
-        return (long long)(t.j1 - t.j0) * (t.diag ? (15 * tl.R) / 2 + 6 : 15 * tl.R + 6);
+        if (!t.diag) return (long long)(t.j1 - t.j0) * (15 * tl.R + 6);
+        const long long m0 = (t.j0 - (long long)t.row * B) / 64, m1 = (t.j1 - (long long)t.row * B) / 64;
+        return 64 * (15 * (m1 * (m1 + 1) - m0 * (m0 + 1)) / 2 + 6 * (m1 - m0));
     };
     std::stable_sort(tasks.begin(), tasks.end(),
                      [&](const MutualTask& a, const MutualTask& b) { return cost(a) > cost(b); });
@@ -641,7 +657,7 @@ static int check_plan_indexes(const Plan& p) {
         GI_CHECK(t.j0 % 64 == 0 && t.j1 % 64 == 0 && 0 <= t.j0 && t.j0 < t.j1 && t.j1 <= sd.gn, "task j-range");
         const long long ib = (long long)t.row * B;
         if (t.diag) {
-            GI_CHECK(t.j0 == ib && t.j1 <= ib + B, "diagonal task j-range");
+            GI_CHECK(t.j0 >= ib && t.j0 < t.j1 && t.j1 <= ib + B, "diagonal task j-range");
             return 0;
         }
         const long long first = ib + B;

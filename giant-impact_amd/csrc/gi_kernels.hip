@@ -232,9 +232,11 @@ __device__ __forceinline__ void mutual_wave_task(const MutualTask& t, const SimD
         // below), and against sub-block m itself over rotations 1..32 (lanes
         // 0..31 only at 32): every pair of the block exactly once, the self
         // pair never.  The j-side sums return to their lanes after 64
-        // rotations and belong to sub-block m's particles.
-        const int nb = (t.j1 - t.j0) >> 6;
-        for (int m = 0; m < nb; ++m) {
+        // rotations and belong to sub-block m's particles.  A task takes the
+        // blocks m of [j0, j1) (the planner may cut a row's diagonal in
+        // pieces of about equal work, each with its own slot).
+        const int m0 = (t.j0 - ib) >> 6, m1 = (t.j1 - ib) >> 6;
+        for (int m = m0; m < m1; ++m) {
             const float4 own = P[ib + m * 64 + lane];
             wave_lds_sync();  // the previous block's reads are done
             q[63 - lane] = own;
