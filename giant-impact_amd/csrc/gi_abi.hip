@@ -277,14 +277,15 @@ constexpr long long kMutualMaxChunk = 4096;
 // last of its group, tasks against the rest of its group (per material
 // segment, cut into pieces of at most L too: uncut, a row against up to
 // 3 x 512 j was the longest task of a small simulation -- one 8,192-particle
-// simulation in the batched engine stepped in 0.41 ms uncut, 0.24 cut).  A
-// row's i-side slots are
-// its group's pieces in order, then its diagonal, then its in-group tasks:
-// contiguous, as the sums expect.  In the batched engine each simulation's L
-// is the engine's, shortened to N / kBatchedPieceDiv (whole 64-blocks, at
-// least one) for simulations too small to fill the GPU with pieces of the
-// full length: a function of the simulation alone, so its sums are the same
-// in any batch.
+// simulation in the batched engine stepped in 0.41 ms uncut, 0.24 cut).  The
+// diagonal is cut likewise, into ranges of the row's 64-blocks of about the
+// work of a piece of L.  A row's i-side slots are its group's pieces in
+// order, then its diagonal pieces, then its in-group tasks: contiguous, as
+// the sums expect.  In the batched engine each simulation's L is the
+// engine's, shortened to N / kBatchedPieceDiv (whole 64-blocks, at least
+// one) for simulations too small to fill the GPU with pieces of the full
+// length: a function of the simulation alone, so its sums are the same in
+// any batch.
 constexpr long long kBatchedPieceDiv = 32;
 
 static int plan_mutual_groups(const std::vector<SimDesc>& sims, const Tiling& tl, long long L0, int W, Plan& p,
