@@ -726,6 +726,37 @@ static int check_plan_indexes(const Plan& p) {
             }
         }
     }
+    // Coverage: every (row, 64-block) pair with a real particle in the block,
+    // at or beyond the row's first block, is taken by exactly one task (its
+    // diagonal pieces, its row or in-group tasks, its group's pieces), and no
+    // task takes any other (simulations of at most 2^26 such pairs).
+    for (size_t si = 0; si < p.sims.size(); ++si) {
+        const SimDesc& sd = p.sims[si];
+        if (sd.gn == 0) continue;
+        const long long nrows = sd.gn / B, nb = sd.gn / 64;
+        if (nrows * nb > (1LL << 26)) continue;
+        const long long fe_real = (sd.fe_end - sd.start + 63) / 64 * 64;
+        const long long si_real = sd.gfe + (sd.end - sd.fe_end + 63) / 64 * 64;
+        std::vector<unsigned char> cov((size_t)(nrows * nb), 0);
+        auto mark = [&](long long row, long long j0, long long j1) -> int {
+            for (long long b = j0 / 64; b < j1 / 64; ++b)
+                GI_CHECK(++cov[(size_t)(row * nb + b)] == 1, "sim %zu row %lld block %lld taken twice", si, row, b);
+            return 0;
+        };
+        for (const MutualTask& t : p.mtasks)
+            if (t.sim == (int)si && t.j1 > t.j0 && (rc = mark(t.row, t.j0, t.j1))) return rc;
+        for (const GroupTask& g : p.gtasks)
+            if (g.kind == 0 && g.sim == (int)si)
+                for (long long rr = (long long)g.group * W; rr < std::min(((long long)g.group + 1) * W, nrows); ++rr)
+                    if ((rc = mark(rr, g.j0, g.j1))) return rc;
+        for (long long rr = 0; rr < nrows; ++rr)
+            for (long long b = 0; b < nb; ++b) {
+                const long long j = 64 * b;
+                const bool want = (j < fe_real || (j >= sd.gfe && j < si_real)) && j >= rr * B;
+                GI_CHECK(cov[(size_t)(rr * nb + b)] == (want ? 1 : 0), "sim %zu row %lld block %lld: taken %d times",
+                         si, rr, b, (int)cov[(size_t)(rr * nb + b)]);
+            }
+    }
 #undef GI_CHECK
     return 0;
 }
