@@ -249,8 +249,9 @@ def test_mutual_plans_stay_in_bounds(gi):
     the row-group kernel, then the per-particle sums) lies inside the buffers
     the plan sizes, and every (row, 64-block) pair holding a real particle is
     taken by exactly one task: ragged, one-material and multi-simulation
-    cases at every row size and chunk length, the batched engine's
-    per-simulation pieces included (host only, gi_plan_check)."""
+    cases at every row size and chunk length, the single-simulation plans'
+    half-length tail pieces and the batched engine's per-simulation pieces
+    included (host only, gi_plan_check)."""
     lib = gi.load()
     rng = np.random.default_rng(7)
     cases = [([1], [0]), ([2], [1]), ([65], [0]), ([257], [100]), ([1100], [512]), ([2100], [1600]),
@@ -262,7 +263,7 @@ def test_mutual_plans_stay_in_bounds(gi):
     for ns, nfes in cases:
         a, b = (C.c_int * len(ns))(*ns), (C.c_int * len(ns))(*nfes)
         for r, v in ((8, 4), (4, 4), (8, 2), (4, 2), (1, 2)):
-            for L, batched in ((64, 0), (1024, 1), (192, 1)):
+            for L, batched in ((64, 0), (256, 0), (1536, 0), (1024, 1), (192, 1)):
                 rc = lib.gi_plan_check(len(ns), a, b, r, v, C.c_longlong(L), batched)
                 assert rc == 0, (ns, nfes, r, v, L, lib.gi_last_error_string())
     assert lib.gi_plan_check(1, (C.c_int * 1)(10), (C.c_int * 1)(3), 2, 4, C.c_longlong(64), 0) == -1  # R >= 4
