@@ -554,8 +554,8 @@ static int build_plan(std::vector<SimDesc> sims, const Tiling& tl, bool batched,
         if (rc < 0) return rc;
     }
     if (tl.variant == 3) {
-        // R = lanes per i-particle (j-streams per wavefront): 1, 2 or 4
-        p.direct_lanes = std::min(tl.R, 4);
+        // R = lanes per i-particle (j-streams per wavefront): 1, 2, 4 or 8
+        p.direct_lanes = std::min(tl.R, 8);
         const int ib = 64 / p.direct_lanes;
         for (size_t si = 0; si < sims.size(); ++si)
             if (sims[si].active)
@@ -1039,7 +1039,9 @@ template <bool POT, int MOVE>
 static void direct_kernel(int lanes, dim3 grid, dim3 block, hipStream_t st, const float4* pos, const float4* vel,
                           const Workspace* ws, float4* force, double* pe, const FusedMove& mv) {
     const DirectBlock* bl = ws->dblocks.p;
-    if (lanes == 4)
+    if (lanes == 8)
+        hipLaunchKernelGGL((k_direct<POT, 8, MOVE>), grid, block, 0, st, pos, vel, bl, ws->sims.p, force, pe, mv);
+    else if (lanes == 4)
         hipLaunchKernelGGL((k_direct<POT, 4, MOVE>), grid, block, 0, st, pos, vel, bl, ws->sims.p, force, pe, mv);
     else if (lanes == 2)
         hipLaunchKernelGGL((k_direct<POT, 2, MOVE>), grid, block, 0, st, pos, vel, bl, ws->sims.p, force, pe, mv);
@@ -1409,7 +1411,7 @@ namespace gi {
 // ---------------------------------------------------------------------------
 
 constexpr int kNoAllocMaxN = 12288;                    // particles per call, at most
-constexpr int kNoAllocMaxBlocks = kNoAllocMaxN / 16;  // >= 16 i-particles per workgroup
+constexpr int kNoAllocMaxBlocks = kNoAllocMaxN / 8;  // >= 8 i-particles per workgroup
 struct NoAllocScratch {
     SimDesc sim;
     DirectBlock blocks[kNoAllocMaxBlocks];
@@ -1696,16 +1698,22 @@ int gi_force_collision_noalloc(const gi_float4* pos, const gi_float4* vel, gi_fl
     // The direct pass with the tuning table's lanes per particle for this n
     // (the plan gi_force_collision makes where the table picks that pass).
     const Tiling t = choose_tiling(n, (long long)n * n, 1);
-    const int lanes = t.variant == 3 ? std::min(t.R, 4) : 1;
+    const int lanes = t.variant == 3 ? std::min(t.R, 8) : 1;
     const int ib = 64 / lanes;
     const int nb = nblk(n, ib);
+    if (nb > kNoAllocMaxBlocks)
+        return set_error(GI_E_TOO_LARGE, "gi_force_collision_noalloc: %d workgroups exceed its maximum of %d", nb,
+                         kNoAllocMaxBlocks);
     hipStream_t st = as_stream(stream);
     hipLaunchKernelGGL(k_noalloc_plan, dim3(nblk(nb, 256)), dim3(256), 0, st, sp, make_sim(0, c->NFe, n, law_of(c)),
                        nb, ib);
     const dim3 grid(nb), block(64 * kDirectWaves);
     const float4 *P = (const float4*)pos, *V = (const float4*)vel;
     float4* F = (float4*)force;
-    if (lanes == 4)
+    if (lanes == 8)
+        hipLaunchKernelGGL((k_direct<false, 8, 0>), grid, block, 0, st, P, V, sp->blocks, &sp->sim, F, nullptr,
+                           FusedMove{});
+    else if (lanes == 4)
         hipLaunchKernelGGL((k_direct<false, 4, 0>), grid, block, 0, st, P, V, sp->blocks, &sp->sim, F, nullptr,
                            FusedMove{});
     else if (lanes == 2)

@@ -1109,7 +1109,9 @@ __global__ __launch_bounds__(64 * kContactWaves) void k_contact_corr(int ntot, c
 // cut into groups of four j dealt out in turn (stream k takes groups k,
 // k + 16 S, ...).  Dealing keeps the contact work -- concentrated on the j
 // near the block's own particles -- spread over all streams; S > 1 gives small
-// simulations S times more wavefronts, each with a shorter chain.  With S = 1
+// simulations S times more wavefronts, each with a shorter chain (S = 8: 17%
+// faster steps at 2,048 particles than S = 4, whose workgroups fill only half
+// of the CUs there).  With S = 1
 // the j come through wave-uniform scalar loads.  Every pair takes the
 // all-pairs pass's branch-free gravity term (float tiles of kDirectTile j
 // folded into a compensated float sum); when some lane of a group has a pair
@@ -1162,7 +1164,7 @@ __global__ __launch_bounds__(64 * kDirectWaves) void k_direct(const float4* __re
                                                               float4* __restrict__ force,
                                                               double* __restrict__ pe, FusedMove mv) {
     static_assert(MOVE == 0 || !POT, "the potential pass moves nothing");
-    static_assert(S == 1 || S == 2 || S == 4, "lanes per particle");
+    static_assert(S == 1 || S == 2 || S == 4 || S == 8, "lanes per particle");
     constexpr int IB = 64 / S;               // i-particles per workgroup
     constexpr int NS = kDirectWaves * S;     // j-streams per particle
     constexpr int TG = kDirectTile / 4;      // groups per stream per float tile
@@ -1796,6 +1798,7 @@ GI_GRAVITY_INST(8, true, 0)
 GI_DIRECT_INST_S(1)
 GI_DIRECT_INST_S(2)
 GI_DIRECT_INST_S(4)
+GI_DIRECT_INST_S(8)
 GI_MUTUAL_INST(1, false)
 GI_MUTUAL_INST(2, false)
 GI_MUTUAL_INST(4, false)

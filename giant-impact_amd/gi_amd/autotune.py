@@ -147,7 +147,7 @@ def main(argv=None) -> int:
         cands = [(R, v, 0, tt, mc) for R, v, tt in itertools.product(Rs, variants, targets)]
         cands += [(R, v, L, 0, 64) for R, L, v in itertools.product([4, 8], mutual_Ls, [2, 4])]
         if n_max <= DIRECT_MAX_CLASS:
-            cands += [(S, 3, 0, 0, 0) for S in (1, 2, 4)]  # R = lanes per i-particle
+            cands += [(S, 3, 0, 0, 0) for S in (1, 2, 4, 8)]  # R = lanes per i-particle
         best = None
         for cand in cands:
             call("gi_set_tiling", *cand)

@@ -178,7 +178,7 @@ int gi_force_collision(const gi_float4 *pos, const gi_float4 *vel, gi_float4 *fo
  * loops, harnesses that forbid hipMalloc): the same force, bitwise, where the
  * tuning table gives gi_force_collision the direct pass (n <= 12,288), with
  * the direct pass's scratch (the simulation descriptor and the block table,
- * ~6 KB) in a static device array that each device holds until the process
+ * ~13 KB) in a static device array that each device holds until the process
  * ends.  Allocates nothing, host or device.  n > 12,288 returns
  * GI_E_TOO_LARGE.  Every call on one device shares that array: calls that
  * overlap on one device (several host threads, or several streams without
@@ -353,7 +353,8 @@ int gi_find_earth_moon(const gi_float4 *pos, const gi_float4 *vel, int n, int nf
  * lane (1, 2, 4 or 8), variant = the all-pairs pass (0: one-sided, r^-3 as rsq
  * cubed; 1: one-sided, rsq*rcp; 2: mutual -- each pair evaluated once for both
  * particles, rows of 64*r particles; 3: direct -- small simulations, the whole
- * force, contacts included, in one launch, r = lanes per i-particle),
+ * force, contacts included, in one launch, r = lanes per i-particle: 1, 2,
+ * 4 or 8),
  * chunk_len = j-particles per task (0 =
  * derived from target_tasks),
  * target_tasks = wavefront tasks to aim for, min_chunk = lower bound of

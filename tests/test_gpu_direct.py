@@ -1,4 +1,4 @@
-"""The direct pass (tiling variant 3, with 1, 2 or 4 lanes per i-particle):
+"""The direct pass (tiling variant 3, with 1, 2, 4 or 8 lanes per i-particle):
 the whole force of a small simulation, contacts included, in one launch.  Checked against the oracle for the impact
 and generator forms, ragged and one-material sizes, a state with receding
 deep contacts, the potential energy, agreement with the three-pass forms,
@@ -17,9 +17,9 @@ from parity import assert_force_parity, force_metrics
 pytestmark = pytest.mark.gpu
 
 
-@pytest.fixture(params=[1, 2, 4], ids=lambda s: f"lanes{s}")
+@pytest.fixture(params=[1, 2, 4, 8], ids=lambda s: f"lanes{s}")
 def direct(gpu, request):
-    """The direct pass with 1, 2 or 4 lanes (j-streams) per i-particle."""
+    """The direct pass with 1, 2, 4 or 8 lanes (j-streams) per i-particle."""
     gpu.call("gi_set_tiling", request.param, 3, 0, 0, 0)
     yield request.param
     gpu.call("gi_set_tiling", 0, 0, 0, 0, 0)
